@@ -1,0 +1,190 @@
+"""Benchmark of the gfx950 Euler step: 2D 4096^2 FP64 PLM+HLLE SSP-RK2.
+
+Workload: planar Sod on [0,1]^2, 4096 x 4096 cells, seeded +-1e-3 density
+jitter, outflow boundaries, decomposed into 16 x 16 patches of 256^2 with
+2-cell halos exchanged on the device, CFL 0.4, device-resident dt. One step =
+one SSP-RK2 step = two cell-updates per cell (the reference's "Updates/sec
+(x2 for RK2)", examples/benchmark/benchmark_euler1d.cpp:217-218).
+
+Timed region: K steps of [dt finalize, halo fill U, fused stage 1, halo fill
+U1, fused stage 2 + RK2 combine + wave-speed reduction], inputs resident in
+HBM, bracketed by stream synchronisation. HIP events on the stream the
+kernels run on give each kernel's duration for the roofline line.
+
+Prints ONE JSON line (rank 0, N=1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "FP64 Mcell-updates/s, 2D 4096² PLM+HLLE RK2; % HBM peak; adjoint/fwd cost"
+N, NP = 4096, 16
+BYTES_STAGE1 = 64.0  # read U^n, write U1 (4 vars x 8 B each)
+BYTES_STAGE2 = 96.0  # read U1, read U^n, write U^{n+1}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(G, budget_s: float):
+    """Oracle (CPU restatement of the reference numerics, OpenMP over patch
+    rows) on the same 4096^2 workload, timed over whole RK2 steps until the
+    time budget is spent."""
+    import numpy as np
+
+    from oracle.oracle import MeshSpec, Oracle
+
+    o = Oracle()
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    o.set_threads(threads)
+    m = MeshSpec(N, N, NP, NP)
+    U = m.to_patches(G)
+    o.mesh_fill(U, m)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        dt = o.mesh_dt(U, m, 1.0 / N, 1.0 / N, 0.4)
+        o.mesh_rk2_step(U, m, dt, 1.0 / N, 1.0 / N)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 20:
+            break
+    val = N * N * 2.0 * steps / el / 1e6
+    del U
+    _ = np
+    return {"value": round(val, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK2 step(s) of the same 4096^2 16x16-patch Sod workload (dt from the "
+                      f"state each step), {el:.1f} s wall on {threads} OpenMP threads"}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic_stage2d_sod4096.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--wpb", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--per-patch-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (single-GPU tier); "
+              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from nest_amd import Sim2D, problems as P
+
+    log(f"bench: building 4096^2 Sod state ({args.mode})")
+    G = P.sod2d(N, N, jitter=1e-3, seed=2)
+    sim = Sim2D.mesh(N, N, NP, NP, mode=args.mode)
+    if args.rows or args.wpb:
+        sim.set_tiling(args.rows, args.wpb, 0)
+    U = sim.scatter(G)
+    sim.upload(U)
+    sim.fill_halos()
+    sim.start(cfl=0.4)
+    log("bench: warmup")
+    sim.advance(args.warmup)
+    sim.synchronize()
+
+    log(f"bench: timed {args.steps} steps")
+    sim.synchronize()
+    t0 = time.perf_counter()
+    comp_ms, ev_total_ms = sim.advance_timed(args.steps)  # synchronises at the end
+    wall = time.perf_counter() - t0
+    st = sim.status()
+    assert st["done"][0] == 0 and st["steps"][0] == args.warmup + args.steps
+
+    ms_per_step = wall * 1e3 / args.steps
+    cells = float(N * N)
+    value = cells * 2.0 * args.steps / wall / 1e6
+    st1_ms = comp_ms[2] / args.steps
+    st2_ms = comp_ms[4] / args.steps
+    fill_ms = (comp_ms[1] + comp_ms[3]) / args.steps
+    # dominant kernel: the fused stage kernel (stage 1 + stage 2 launches)
+    alg_bytes = (BYTES_STAGE1 + BYTES_STAGE2) * cells
+    achieved = alg_bytes / ((st1_ms + st2_ms) * 1e-3) / 1e9
+    step_gbs = alg_bytes / (ms_per_step * 1e-3) / 1e9
+
+    # graph-replayed variant of the same steps
+    log("bench: graph replay")
+    sim.advance(2, graph=True)
+    sim.synchronize()
+    tg = time.perf_counter()
+    sim.advance(args.steps, graph=True)
+    sim.synchronize()
+    graph_wall = time.perf_counter() - tg
+    graph_value = cells * 2.0 * args.steps / graph_wall / 1e6
+
+    # plain per-patch drop-in path (host dt, one launch set per patch)
+    pp = None
+    if args.per_patch_steps > 0:
+        log("bench: per-patch path")
+        sim.advance_per_patch(1, 0.4)
+        sim.synchronize()
+        tp = time.perf_counter()
+        sim.advance_per_patch(args.per_patch_steps, 0.4)
+        sim.synchronize()
+        pp_wall = time.perf_counter() - tp
+        pp = round(cells * 2.0 * args.per_patch_steps / pp_wall / 1e6, 1)
+    sim.close()
+    del U
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline(G, args.cpu_budget)
+
+    traffic = load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded planar Sod with +-1e-3 density jitter)",
+        "config": {"workload": "2D planar Sod 4096x4096 FP64, 16x16 patches of 256^2, halo 2, outflow, "
+                               "PLM(minmod)+HLLE, SSP-RK2, CFL 0.4, device dt",
+                   "numerics": args.mode, "cells": N * N, "patches": NP * NP},
+        "mcell_steps_per_s": round(value / 2.0, 1),
+        "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "kernel_ms": {"stage1": round(st1_ms, 4), "stage2": round(st2_ms, 4), "halo_fill": round(fill_ms, 4),
+                      "finalize": round(comp_ms[0] / args.steps, 4)},
+        "graph_value": round(graph_value, 1),
+        "per_patch_value": pp,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (traffic or {}).get("bytes_per_launch")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

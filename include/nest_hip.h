@@ -1,0 +1,170 @@
+/*
+ * nest_hip.h — C ABI of the MI355X (gfx950) backend for nest's Euler
+ * finite-volume step: fused PLM + HLLE + conservative update (1D and 2D),
+ * SSP-RK2 stages, on-device patch halo fill and the CFL wave-speed
+ * reduction, as hand-written HIP kernels.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; no C++ or torch types cross this ABI.
+ *  - Every array is in nest's field_t SoA layout (reference
+ *    include/nest/core.hpp:206-221, :310-392): data[var*size(ext) + linear],
+ *    linear index row-major with the LAST index contiguous, over the patch's
+ *    extended space (interior + 2-cell halo on every side). 1D: 3 variables
+ *    (rho, rho*u, E); 2D: 4 variables (rho, rho*u, rho*v, E) with u along
+ *    dim 0 (x) and v along dim 1 (y, contiguous).
+ *  - Multi-patch buffers hold every patch's array back to back in patch-index
+ *    order (patch index = py*npx + px, reference patch.hpp:219-229).
+ *  - Functions return NEST_OK (0) or an error code; nest_hip_last_error()
+ *    gives a message (thread-local). No exception crosses the ABI.
+ *  - A handle owns its device buffers and one HIP stream (default priority)
+ *    unless the caller supplies a stream. A handle is not thread-safe;
+ *    different handles may be used concurrently.
+ *  - Numerics mode: NEST_MODE_EXACT reproduces the reference CPU arithmetic
+ *    bit for bit (IEEE division/sqrt, no FMA contraction); NEST_MODE_FAST
+ *    uses reciprocal-multiply and FMA (agrees to ~1e-15 relative per step).
+ */
+#ifndef NEST_HIP_H
+#define NEST_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nest_status {
+    NEST_OK = 0,
+    NEST_ERR_INVALID = 1,     /* bad argument / handle */
+    NEST_ERR_ALLOC = 2,       /* device allocation failed */
+    NEST_ERR_HIP = 3,         /* HIP runtime error */
+    NEST_ERR_NONFINITE = 4,   /* a problem produced a non-finite dt */
+    NEST_ERR_UNSUPPORTED = 5  /* configuration not supported */
+};
+enum nest_bc { NEST_BC_PERIODIC = 0, NEST_BC_OUTFLOW = 1 };
+/* limiter_t of reference examples/sod1d/euler1d.hpp:106-110 */
+enum nest_limiter { NEST_LIM_MINMOD = 0, NEST_LIM_MC = 1, NEST_LIM_NONE = 2 };
+enum nest_mode { NEST_MODE_EXACT = 0, NEST_MODE_FAST = 1 };
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* nest_hip_last_error(void);
+/* ABI version (major*100 + minor). */
+int nest_hip_version(void);
+/* Number of visible HIP devices (0 if none); does not create a context. */
+int nest_hip_device_count(void);
+
+/* ===================================================================== */
+/* 1D single-patch solver. Replaces the CUDA solver object of reference   */
+/* src/backend_cuda/euler1d_solver.cu:35-189 (behind euler1d_solver.hpp). */
+/* ===================================================================== */
+typedef struct nest_solver1d nest_solver1d;
+
+/* n interior cells, halo 2, cell size dx, gamma, bc (periodic/outflow),
+   limiter, mode. Replaces euler1d_gpu_create (euler1d_solver.hpp:27). */
+int nest_hip_solver1d_create(int n, double dx, double gamma, int bc, int limiter, int mode,
+                             nest_solver1d** out);
+int nest_hip_solver1d_destroy(nest_solver1d* s); /* euler1d_solver.hpp:32 */
+/* host buffer of 3*(n+4) doubles, field_t layout (euler1d_solver.hpp:41,49) */
+int nest_hip_solver1d_upload(nest_solver1d* s, const double* host);
+int nest_hip_solver1d_download(nest_solver1d* s, double* host);
+/* dt = cfl*dx/max(|u|+c) of the current state; blocking
+   (euler1d_solver.hpp:58, euler1d_solver.cu:100-119). */
+int nest_hip_solver1d_compute_dt(nest_solver1d* s, double cfl, double* dt);
+/* One SSP-RK2 step with the given dt (euler1d_solver.hpp:66). */
+int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt);
+/* Run to t_final with dt = min(cfl*dx/max(|u|+c), t_final - t) computed on
+   the device every step; the host only polls completion every batch of
+   steps (euler1d_solver.hpp:76). *steps receives the step count. */
+int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* steps);
+/* Use the caller's stream (hipStream_t, NULL = handle's own). */
+int nest_hip_solver1d_set_stream(nest_solver1d* s, void* stream);
+
+/* ---- reference-compatible entry points: exact signatures of reference
+   src/backend_cuda/euler1d_solver.hpp:16-87 (periodic BC, minmod, halo 2,
+   exact numerics). Errors print to stderr and return NULL / -1 / NaN
+   instead of throwing through extern "C". ---- */
+struct euler1d_gpu_t;
+struct euler1d_gpu_t* euler1d_gpu_create(int n_interior, int n_halo, double dx, double gamma);
+void euler1d_gpu_destroy(struct euler1d_gpu_t* solver);
+void euler1d_gpu_upload(struct euler1d_gpu_t* solver, const double* h_data);
+void euler1d_gpu_download(struct euler1d_gpu_t* solver, double* h_data);
+double euler1d_gpu_compute_dt(struct euler1d_gpu_t* solver, double cfl);
+void euler1d_gpu_rk2_step(struct euler1d_gpu_t* solver, double dt);
+int euler1d_gpu_run(struct euler1d_gpu_t* solver, double t_final, double cfl);
+double euler1d_gpu_run_timed(struct euler1d_gpu_t* solver, double t_final, double cfl, int* steps);
+
+/* ===================================================================== */
+/* 2D simulations: one patch mesh, or a batch of independent problems.    */
+/* Replaces the 2D CPU driver loop of reference examples/sod2d/sod2d.cpp: */
+/* 318-356 (exchange -> x_sweep_t -> exchange -> y_sweep_t, euler2d.hpp:  */
+/* 67-239) and compute_dt_2d (euler2d.hpp:242-271), with the 2D step      */
+/* defined as SSP-RK2 over the X-then-Y split stage.                      */
+/* ===================================================================== */
+typedef struct nest_sim2d nest_sim2d;
+
+/* nx*ny cells on npx*npy patches (tensor-product partition with
+   subspace() semantics, reference core.hpp:184-199, patch.hpp:207-229),
+   halo 2, boundary conditions per dimension. Every patch must have at least
+   2 interior cells per dimension. */
+int nest_hip_mesh2d_create(int nx, int ny, int npx, int npy, double dx, double dy, double gamma, int bcx,
+                           int bcy, int limiter, int mode, nest_sim2d** out);
+/* nprob independent single-patch problems of nx[i]*ny[i] cells with cell
+   sizes dx[i], dy[i], all sharing gamma, bc and limiter; stepped together. */
+int nest_hip_batch2d_create(int nprob, const int* nx, const int* ny, const double* dx, const double* dy,
+                            double gamma, int bc, int limiter, int mode, nest_sim2d** out);
+int nest_hip_sim2d_destroy(nest_sim2d* s);
+/* total doubles of the concatenated buffer, patch / problem / work-item counts */
+int nest_hip_sim2d_info(const nest_sim2d* s, size_t* total, int* npatches, int* nprobs, int* nitems);
+/* element offset, interior size and problem index of one patch */
+int nest_hip_sim2d_patch_info(const nest_sim2d* s, int patch, long long* off, int* nx, int* ny, int* prob);
+int nest_hip_sim2d_set_stream(nest_sim2d* s, void* stream);
+/* Operate on caller-owned device buffers (each `total` doubles), e.g. torch
+   tensors; NULL restores the handle's own buffer for that slot. */
+int nest_hip_sim2d_bind(nest_sim2d* s, double* dev_U, double* dev_U1);
+int nest_hip_sim2d_device_ptrs(nest_sim2d* s, double** dev_U, double** dev_U1);
+/* Work decomposition: rows per work item (>= 1) and waves per block (1..16);
+   0 keeps the current value. persistent != 0 caps the grid at the number of
+   resident waves and loops over the work list. */
+int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int persistent);
+/* host <-> device copies of the whole concatenated buffer U (blocking) */
+int nest_hip_sim2d_upload(nest_sim2d* s, const double* host);
+int nest_hip_sim2d_download(nest_sim2d* s, double* host);
+int nest_hip_sim2d_upload_patch(nest_sim2d* s, int patch, const double* host);
+int nest_hip_sim2d_download_patch(nest_sim2d* s, int patch, double* host);
+/* Halo exchange of U: every ghost cell (faces and corners) of every patch
+   from its neighbour or physical boundary (periodic wrap / outflow copy);
+   one launch. Extends execute_all_exchanges (reference patch.hpp:153-190). */
+int nest_hip_sim2d_fill_halos(nest_sim2d* s);
+/* compute_dt_2d per problem into dt_host[nprob] (blocking). */
+int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host);
+/* One split stage with a host dt for every problem:
+   stage 1: U1 = S(U); stage 2: U = 0.5*(U + S(U1)). Halos of the input must
+   be filled. patch < 0: all patches in one launch; else that patch only. */
+int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch);
+/* One full RK2 step with a host dt: fill(U), stage 1, fill(U1), stage 2,
+   fill(U). */
+int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt);
+/* Device-resident time stepping. start(): t = 0, dt policy per problem
+   (fixed_dt > 0 holds dt fixed, else CFL with cfl), stop at t_end (use
+   INFINITY for a fixed step count), and reduces the initial wave speeds.
+   advance(): nsteps steps of [dt finalize, fill, stage 1, fill, stage 2]
+   enqueued without any host synchronisation; use_graph != 0 replays one
+   captured step graph. status(): per-problem t, dt, steps and done flags
+   (arrays of nprob; any may be NULL), blocking. */
+int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_dt);
+int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph);
+int nest_hip_sim2d_status(nest_sim2d* s, double* t, double* dt, int* steps, int* done);
+/* Plain per-patch drop-in path (what a patch-by-patch Pipeline executor
+   does): host dt with a blocking read-back, then per patch one launch per
+   stage. nsteps steps. */
+int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl);
+int nest_hip_sim2d_synchronize(nest_sim2d* s);
+/* advance() without a graph, with HIP events recorded on the handle's stream
+   around each launch of the step; blocks at the end. ms[5] receives the
+   summed durations of [dt finalize, fill U, stage 1, fill U1, stage 2] over
+   the nsteps steps, *total_ms the elapsed time of the whole sequence. */
+int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEST_HIP_H */

@@ -1,0 +1,190 @@
+// Device-side Euler numerics for gfx950 (FP64, wave64).
+//
+// Included by kernels.hip, which is compiled twice:
+//   NEST_FAST=0 (-ffp-contract=off): every operation is the IEEE double
+//     operation the reference CPU code performs, in its order
+//     (examples/sod1d/euler1d.hpp:27-242, examples/sod2d/euler2d.hpp:20-64),
+//     with correctly rounded division and square root, so results are
+//     bitwise those of the CPU path.
+//   NEST_FAST=1 (FMA contraction on): divisions become reciprocal-multiplies
+//     (hardware v_rcp_f64 refined by two Newton steps), per-face quotients
+//     share one reciprocal, and the EOS uses 1/(gamma-1). Differences from
+//     the CPU path are a few ulp per operation; the per-step tolerance is
+//     1e-12 relative.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nest_types.hpp"
+
+#ifndef NEST_FAST
+#error "define NEST_FAST to 0 or 1"
+#endif
+
+namespace NEST_NS {
+
+enum : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_NONE = 2 };
+
+using eos_c = ::nest_eos;  // gamma, gm1 = gamma - 1.0, inv_gm1 = 1.0 / gm1 (fast mode only)
+
+// std::min / std::max semantics (selects on a strict compare)
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+#if NEST_FAST
+// 1/x: hardware estimate + two Newton-Raphson steps (error ~1 ulp)
+__device__ __forceinline__ double rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return y;
+}
+__device__ __forceinline__ double ddiv(double a, double b) { return a * rcp(b); }
+#else
+__device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
+#endif
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+
+// ---------------------------------------------------------------------------
+// limiters — euler1d.hpp:112-137
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double minmod(double a, double b) {
+    double r = (__builtin_fabs(a) < __builtin_fabs(b)) ? a : b;
+    return (a * b <= 0.0) ? 0.0 : r;
+}
+template <int LIM>
+__device__ __forceinline__ double limiter(double l, double r) {
+    if constexpr (LIM == LIM_MINMOD) {
+        return minmod(l, r);
+    } else if constexpr (LIM == LIM_MC) {
+        double c = 0.5 * (l + r);
+        return minmod(2.0 * l, minmod(c, 2.0 * r));
+    } else {
+        return 0.5 * (l + r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// primitive state of one cell, 4 reconstructed quantities
+// 2D: (rho, u_normal, p, q = transverse momentum), euler2d.hpp:36-64
+// 1D: (rho, u, p, unused)
+// ---------------------------------------------------------------------------
+struct prim4 {
+    double r, u, p, q;
+};
+
+// get_prim_2d — euler2d.hpp:44-52. mn = normal momentum, mt = transverse.
+// The kinetic energy always uses u = rho_u/rho, v = rho_v/rho in (u*u + v*v)
+// order, whichever direction is normal, as the reference does.
+__device__ __forceinline__ prim4 prim2d(double r_in, double mu, double mv, double E, bool xsweep,
+                                        const eos_c& e) {
+    prim4 w;
+    double rho = dmax(r_in, 1e-10);
+#if NEST_FAST
+    double ir = rcp(rho);
+    double u = mu * ir;
+    double v = mv * ir;
+#else
+    double u = mu / rho;
+    double v = mv / rho;
+#endif
+    double ke = 0.5 * rho * (u * u + v * v);
+    w.r = rho;
+    w.p = dmax(e.gm1 * (E - ke), 1e-10);
+    w.u = xsweep ? u : v;
+    w.q = xsweep ? mv : mu;
+    return w;
+}
+
+// cons_to_prim — euler1d.hpp:69-84 with eos_t::pressure :31-44 (p >= 0 clamp,
+// no density floor). The reference re-divides m/rho inside pressure(); the
+// quotient is the same value.
+__device__ __forceinline__ prim4 prim1d(double rho, double m, double E, const eos_c& e) {
+    prim4 w;
+    double u = ddiv(m, rho);
+    double ke = 0.5 * rho * u * u;
+    w.r = rho;
+    w.u = u;
+    w.p = dmax(e.gm1 * (E - ke), 0.0);
+    w.q = 0.0;
+    return w;
+}
+
+// sound speed — euler1d.hpp:54-56
+__device__ __forceinline__ double sound_speed(double rho, double p, const eos_c& e) {
+    return dsqrt(ddiv(e.gamma * p, rho));
+}
+// total energy — euler1d.hpp:47-51
+__device__ __forceinline__ double total_energy(double rho, double u, double p, const eos_c& e) {
+    double ke = 0.5 * rho * u * u;
+#if NEST_FAST
+    double ie = p * e.inv_gm1;
+#else
+    double ie = p / e.gm1;
+#endif
+    return ie + ke;
+}
+
+// HLLE flux with Davis speeds (euler1d.hpp:178-242) on (rho, u_n, p), plus the
+// passive transverse-momentum flux upwinded by the HLLE-weighted interface
+// velocity (euler2d.hpp:20-33, :115-122). The sound speeds are shared between
+// the two (the reference recomputes the same values, :116-118).
+// F[0..2] = HLLE (mass, normal momentum, energy); F[3] = passive flux.
+template <bool PASSIVE>
+__device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const eos_c& e, double F[4]) {
+    double cL = sound_speed(L.r, L.p, e);
+    double EL = total_energy(L.r, L.u, L.p, e);
+    double cR = sound_speed(R.r, R.p, e);
+    double ER = total_energy(R.r, R.u, R.p, e);
+    double SL = dmin(L.u - cL, R.u - cR);
+    double SR = dmax(L.u + cL, R.u + cR);
+    double FL0 = L.r * L.u, FL1 = L.r * L.u * L.u + L.p, FL2 = (EL + L.p) * L.u;
+    double FR0 = R.r * R.u, FR1 = R.r * R.u * R.u + R.p, FR2 = (ER + R.p) * R.u;
+    double UL1 = L.r * L.u, UR1 = R.r * R.u;
+    double sLR = SL * SR;
+#if NEST_FAST
+    double inv = rcp(SR - SL);
+    double H0 = (SR * FL0 - SL * FR0 + sLR * (R.r - L.r)) * inv;
+    double H1 = (SR * FL1 - SL * FR1 + sLR * (UR1 - UL1)) * inv;
+    double H2 = (SR * FL2 - SL * FR2 + sLR * (ER - EL)) * inv;
+    double Hu = (SR * L.u - SL * R.u) * inv;
+#else
+    double d = SR - SL;
+    double H0 = (SR * FL0 - SL * FR0 + sLR * (R.r - L.r)) / d;
+    double H1 = (SR * FL1 - SL * FR1 + sLR * (UR1 - UL1)) / d;
+    double H2 = (SR * FL2 - SL * FR2 + sLR * (ER - EL)) / d;
+    double Hu = (SR * L.u - SL * R.u) / d;
+#endif
+    bool left = SL >= 0.0;
+    bool right = !left && (SR <= 0.0);
+    F[0] = left ? FL0 : (right ? FR0 : H0);
+    F[1] = left ? FL1 : (right ? FR1 : H1);
+    F[2] = left ? FL2 : (right ? FR2 : H2);
+    if constexpr (PASSIVE) {
+        double ui = left ? L.u : (right ? R.u : Hu);
+        F[3] = (ui >= 0.0) ? ui * L.q : ui * R.q;
+    } else {
+        (void)Hu;
+        F[3] = 0.0;
+    }
+}
+
+// slope of all 4 quantities from the left and right differences
+template <int LIM>
+__device__ __forceinline__ prim4 slope4(const prim4& dl, const prim4& dr) {
+    return {limiter<LIM>(dl.r, dr.r), limiter<LIM>(dl.u, dr.u), limiter<LIM>(dl.p, dr.p),
+            limiter<LIM>(dl.q, dr.q)};
+}
+__device__ __forceinline__ prim4 sub4(const prim4& a, const prim4& b) {
+    return {a.r - b.r, a.u - b.u, a.p - b.p, a.q - b.q};
+}
+// W + 0.5*s (right face of a cell) / W - 0.5*s (left face), euler1d.hpp:166-167
+__device__ __forceinline__ prim4 face_plus(const prim4& w, const prim4& s) {
+    return {w.r + 0.5 * s.r, w.u + 0.5 * s.u, w.p + 0.5 * s.p, w.q + 0.5 * s.q};
+}
+__device__ __forceinline__ prim4 face_minus(const prim4& w, const prim4& s) {
+    return {w.r - 0.5 * s.r, w.u - 0.5 * s.u, w.p - 0.5 * s.p, w.q - 0.5 * s.q};
+}
+
+}  // namespace NEST_NS

@@ -1,0 +1,429 @@
+// Forward kernels of the Euler finite-volume step for gfx950 (wave64, FP64).
+//
+// Compiled twice into the library: NEST_FAST=0 / namespace nest_exact
+// (-ffp-contract=off, IEEE division and sqrt: bitwise the CPU numerics) and
+// NEST_FAST=1 / namespace nest_fast (FMA contraction, reciprocal division).
+//
+// 2D stage kernel ("strip march"): one wave owns a strip of 64 consecutive
+// extended columns (dim 1, contiguous in memory) of one patch and marches
+// down interior rows [r0, r1) of dim 0. Per row it loads one new row of the
+// conserved state (four coalesced 512 B loads), and:
+//   * X-sweep (along the march): primitives, differences, slopes and the
+//     face flux are carried in registers as a sliding window, so every
+//     cell's primitives and every x-face flux are computed exactly once;
+//   * Y-sweep (across lanes) on the freshly X-swept row: neighbour
+//     primitives, differences, face states and fluxes move between lanes by
+//     wave shuffles; lanes 2..61 own outputs, lanes 0,1,62,63 are the
+//     stencil halo (the X-sweep of the halo columns is exactly what the
+//     split scheme's halo exchange would have delivered);
+//   * epilogue: stage 1 stores U1; stage 2 stores 0.5*(U^n + S(U1)) in place
+//     of U^n and folds |u|+c, |v|+c of the new state into the problem's
+//     wave-speed maxima (u64 bit-pattern atomicMax: exact, order-free).
+// No LDS is used and nothing but the state itself touches HBM.
+#include "device_numerics.hpp"
+#include "kernels.hpp"
+
+namespace NEST_NS {
+
+__device__ __forceinline__ double ld(const double* p) { return *p; }
+
+__device__ __forceinline__ prim4 shfl_up4(const prim4& a) {
+    return {__shfl_up(a.r, 1), __shfl_up(a.u, 1), __shfl_up(a.p, 1), __shfl_up(a.q, 1)};
+}
+__device__ __forceinline__ prim4 shfl_down4(const prim4& a) {
+    return {__shfl_down(a.r, 1), __shfl_down(a.u, 1), __shfl_down(a.p, 1), __shfl_down(a.q, 1)};
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long w = __shfl_xor(v, o);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
+// |u|+c and |v|+c of one cell, as compute_dt_2d (euler2d.hpp:254-263)
+__device__ __forceinline__ void cell_speeds2d(double r, double mu, double mv, double E, const eos_c& e,
+                                              double& sx, double& sy) {
+    double rho = dmax(r, 1e-10);
+    double u = ddiv(mu, rho);
+    double v = ddiv(mv, rho);
+    double ke = 0.5 * rho * (u * u + v * v);
+    double p = dmax(e.gm1 * (E - ke), 1e-10);
+    double c = sound_speed(rho, p, e);
+    sx = __builtin_fabs(u) + c;
+    sy = __builtin_fabs(v) + c;
+}
+
+template <int LIM, int STAGE>
+__global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+        const nest_item w = a.items[it];
+        const nest_patch_desc pd = a.patches[w.patch];
+        nest_prob_state* ps = a.probs + pd.prob;
+        if (!(a.host_dt > 0.0) && ps->done) continue;
+        const double dt = a.host_dt > 0.0 ? a.host_dt : ps->dt;
+        const double dtdx = dt / ps->dx;
+        const double dtdy = dt / ps->dy;
+
+        const int nye = pd.ny + 4;
+        const int vs = (pd.nx + 4) * nye;
+        const int ec = w.c0 + lane;
+        const int ecl = ec < nye ? ec : nye - 1;
+        const double* in = (STAGE == 1 ? a.U : a.U1) + pd.off + ecl;
+        double* out = (STAGE == 1 ? a.U1 : a.U) + pd.off + ec;
+        const bool store = lane >= 2 && lane < 62 && ec >= 2 && ec < pd.ny + 2;
+
+        auto load_row = [&](int e, double* u) {
+            const double* p = in + (size_t)e * nye;
+            u[0] = ld(p);
+            u[1] = ld(p + vs);
+            u[2] = ld(p + 2 * (size_t)vs);
+            u[3] = ld(p + 3 * (size_t)vs);
+        };
+
+        // ---- X window warm-up: rows i = r0-2 .. r0+1 (extended rows r0 .. r0+3)
+        double A[4], B[4], Ua[4], Ub[4];
+        load_row(w.r0, A);
+        load_row(w.r0 + 1, B);
+        load_row(w.r0 + 2, Ua);
+        load_row(w.r0 + 3, Ub);
+        prim4 Wm2 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+        prim4 Wm1 = prim2d(B[0], B[1], B[2], B[3], true, a.eos);
+        prim4 W0 = prim2d(Ua[0], Ua[1], Ua[2], Ua[3], true, a.eos);
+        prim4 W1 = prim2d(Ub[0], Ub[1], Ub[2], Ub[3], true, a.eos);
+        prim4 dm1 = sub4(Wm1, Wm2);
+        prim4 d0 = sub4(W0, Wm1);
+        prim4 dW1 = sub4(W1, W0);
+        prim4 s_m1 = slope4<LIM>(dm1, d0);
+        prim4 s0 = slope4<LIM>(d0, dW1);
+        double Fm[4];
+        face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+
+        unsigned long long smx = 0, smy = 0;
+        for (int i = w.r0; i < w.r1; ++i) {
+            double Un[4];
+            load_row(i + 4, Un);
+            prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
+            prim4 dW2 = sub4(W2, W1);
+            prim4 s1 = slope4<LIM>(dW1, dW2);
+            double Fp[4];
+            face_flux<true>(face_plus(W0, s0), face_minus(W1, s1), a.eos, Fp);
+            // x_sweep_t update (euler2d.hpp:144-148)
+            const double ux0 = Ua[0] - dtdx * (Fp[0] - Fm[0]);
+            const double ux1 = Ua[1] - dtdx * (Fp[1] - Fm[1]);
+            const double ux2 = Ua[2] - dtdx * (Fp[3] - Fm[3]);
+            const double ux3 = Ua[3] - dtdx * (Fp[2] - Fm[2]);
+            W0 = W1;
+            W1 = W2;
+            dW1 = dW2;
+            s0 = s1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                Fm[k] = Fp[k];
+                Ua[k] = Ub[k];
+                Ub[k] = Un[k];
+            }
+
+            // ---- Y-sweep of the X-swept row across lanes (euler2d.hpp:176-236)
+            prim4 Y = prim2d(ux0, ux1, ux2, ux3, false, a.eos);
+            prim4 dl = sub4(Y, shfl_up4(Y));
+            prim4 dr = shfl_down4(dl);
+            prim4 sy = slope4<LIM>(dl, dr);
+            prim4 Rf = shfl_down4(face_minus(Y, sy));
+            double G[4];
+            face_flux<true>(face_plus(Y, sy), Rf, a.eos, G);
+            double Gm[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Gm[k] = __shfl_up(G[k], 1);
+            double o0 = ux0 - dtdy * (G[0] - Gm[0]);
+            double o1 = ux1 - dtdy * (G[3] - Gm[3]);
+            double o2 = ux2 - dtdy * (G[1] - Gm[1]);
+            double o3 = ux3 - dtdy * (G[2] - Gm[2]);
+            double* po = out + (size_t)(i + 2) * nye;
+            if constexpr (STAGE == 2) {
+                // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2), in place of U^n
+                if (store) {
+                    o0 = 0.5 * (po[0] + o0);
+                    o1 = 0.5 * (po[vs] + o1);
+                    o2 = 0.5 * (po[2 * (size_t)vs] + o2);
+                    o3 = 0.5 * (po[3 * (size_t)vs] + o3);
+                    double sx, sy2;
+                    cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
+                    unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+                    unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
+                    smx = bx > smx ? bx : smx;
+                    smy = by > smy ? by : smy;
+                }
+            }
+            if (store) {
+                po[0] = o0;
+                po[vs] = o1;
+                po[2 * (size_t)vs] = o2;
+                po[3 * (size_t)vs] = o3;
+            }
+        }
+        if constexpr (STAGE == 2) {
+            smx = wave_max_u64(smx);
+            smy = wave_max_u64(smy);
+            if (lane == 0) {
+                atomicMax(&ps->smax[0], smx);
+                atomicMax(&ps->smax[1], smy);
+            }
+        }
+    }
+}
+
+// Wave speeds of the current state (first step of a run): same work items.
+__global__ __launch_bounds__(256) void speeds2d_kernel(stage_args a) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+        const nest_item w = a.items[it];
+        const nest_patch_desc pd = a.patches[w.patch];
+        nest_prob_state* ps = a.probs + pd.prob;
+        const int nye = pd.ny + 4;
+        const int vs = (pd.nx + 4) * nye;
+        const int ec = w.c0 + lane;
+        const bool own = lane >= 2 && lane < 62 && ec >= 2 && ec < pd.ny + 2;
+        unsigned long long smx = 0, smy = 0;
+        if (own) {
+            const double* p0 = a.U + pd.off + ec;
+            for (int i = w.r0; i < w.r1; ++i) {
+                const double* p = p0 + (size_t)(i + 2) * nye;
+                double sx, sy;
+                cell_speeds2d(p[0], p[vs], p[2 * (size_t)vs], p[3 * (size_t)vs], a.eos, sx, sy);
+                unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+                unsigned long long by = (unsigned long long)__double_as_longlong(sy);
+                smx = bx > smx ? bx : smx;
+                smy = by > smy ? by : smy;
+            }
+        }
+        smx = wave_max_u64(smx);
+        smy = wave_max_u64(smy);
+        if (lane == 0) {
+            atomicMax(&ps->smax[0], smx);
+            atomicMax(&ps->smax[1], smy);
+        }
+    }
+}
+
+// Device exchange-plan executor: every region of every patch in one launch.
+__global__ __launch_bounds__(256) void fill_regions_kernel(double* base, const nest_region* R, int nreg,
+                                                           int nvars) {
+    for (int r = blockIdx.x; r < nreg; r += gridDim.x) {
+        const nest_region g = R[r];
+        const int n = g.ni * g.nj;
+        for (int t = threadIdx.x; t < n * nvars; t += blockDim.x) {
+            const int k = t / n;
+            const int c = t - k * n;
+            const int ia = c / g.nj;
+            const int jb = c - ia * g.nj;
+            const int si = g.si + (g.bi ? 0 : ia);
+            const int sj = g.sj + (g.bj ? 0 : jb);
+            base[g.dst_off + (long long)k * g.dst_vs + (long long)(g.di + ia) * g.dst_nye + g.dj + jb] =
+                base[g.src_off + (long long)k * g.src_vs + (long long)si * g.src_nye + sj];
+        }
+    }
+}
+
+// dt for the next step of every problem from the reduced wave speeds:
+// 2D compute_dt_2d (euler2d.hpp:249-270): cfl*min(min dx/sx, min dy/sy),
+//    and min over cells of dx/s equals dx/max s (correctly rounded division
+//    is monotone), init 1e10, cells with s == 0 skipped;
+// 1D compute_dt (sod1d.cpp:127): (cfl*dx)/max(|u|+c);
+// then dt = min(dt, t_end - t) (sod1d.cpp:222, sod2d.cpp:320).
+__global__ void finalize_kernel(nest_prob_state* P, int nprob) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    nest_prob_state& s = P[i];
+    if (s.done) return;
+    if (!(s.t < s.t_end)) {
+        s.done = 1;
+        return;
+    }
+    const double sx = __longlong_as_double((long long)s.smax[0]);
+    const double sy = __longlong_as_double((long long)s.smax[1]);
+    double dt;
+    if (s.fixed_dt > 0.0) {
+        dt = s.fixed_dt;
+    } else if (s.dim == 1) {
+        dt = s.cfl * s.dx / sx;
+    } else {
+        double dtx = 1e10, dty = 1e10;
+        if (sx > 0.0) dtx = dmin(dtx, s.dx / sx);
+        if (sy > 0.0) dty = dmin(dty, s.dy / sy);
+        dt = s.cfl * dmin(dtx, dty);
+    }
+    dt = dmin(dt, s.t_end - s.t);
+    if (!(dt > 0.0) || dt != dt || dt == __builtin_inf() || sx != sx || sy != sy) {
+        s.done = 2;
+        s.dt = dt;
+        return;
+    }
+    s.dt = dt;
+    s.t = s.t + dt;
+    s.steps += 1;
+    s.smax[0] = 0ull;
+    s.smax[1] = 0ull;
+}
+
+__global__ void reset_speeds_kernel(nest_prob_state* P, int nprob) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nprob) {
+        P[i].smax[0] = 0ull;
+        P[i].smax[1] = 0ull;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1D (single patch, n interior cells, halo 2, 3 variables)
+// Block of 256 threads owns 256 consecutive interior cells; primitives,
+// slopes and the 257 face fluxes of the block are staged in LDS so each is
+// computed once (benchmark_euler1d.cpp:113-165 numerics).
+// ---------------------------------------------------------------------------
+template <int LIM, int STAGE>
+__global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
+    __shared__ prim4 W[260];
+    __shared__ prim4 S[258];
+    __shared__ double F[3][257];
+    if (!(a.host_dt > 0.0) && a.ps->done) return;
+    const int n = a.n, ne = n + 4;
+    const int base = 2 + blockIdx.x * 256;  // first extended cell of the block
+    const double* in = STAGE == 1 ? a.U : a.U1;
+    double* out = STAGE == 1 ? a.U1 : a.U;
+    for (int t = threadIdx.x; t < 260; t += 256) {
+        int e = base - 2 + t;
+        e = e < ne ? e : ne - 1;
+        W[t] = prim1d(in[e], in[ne + e], in[2 * ne + e], a.eos);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 258; t += 256) {  // slopes of cells base-1+t
+        S[t] = slope4<LIM>(sub4(W[t + 1], W[t]), sub4(W[t + 2], W[t + 1]));
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 257; t += 256) {  // face between cells base-1+t and base+t
+        double f[4];
+        face_flux<false>(face_plus(W[t + 1], S[t]), face_minus(W[t + 2], S[t + 1]), a.eos, f);
+        F[0][t] = f[0];
+        F[1][t] = f[1];
+        F[2][t] = f[2];
+    }
+    __syncthreads();
+    const int e = base + threadIdx.x;
+    unsigned long long sm = 0;
+    if (e < n + 2) {
+        const double dt = a.host_dt > 0.0 ? a.host_dt : a.ps->dt;
+        const double dtdx = dt / a.ps->dx;
+        const int t = threadIdx.x;
+        double o[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[k] = in[k * ne + e] - dtdx * (F[k][t + 1] - F[k][t]);
+        if constexpr (STAGE == 2) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) o[k] = 0.5 * (out[k * ne + e] + o[k]);
+            prim4 w = prim1d(o[0], o[1], o[2], a.eos);
+            double s = __builtin_fabs(w.u) + sound_speed(w.r, w.p, a.eos);
+            sm = (unsigned long long)__double_as_longlong(s);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
+    }
+    if constexpr (STAGE == 2) {
+        sm = wave_max_u64(sm);
+        if ((threadIdx.x & 63) == 0) atomicMax(&a.ps->smax[0], sm);
+    }
+}
+
+__global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
+    const int n = a.n, ne = n + 4;
+    const int e = 2 + blockIdx.x * 256 + threadIdx.x;
+    unsigned long long sm = 0;
+    if (e < n + 2) {
+        prim4 w = prim1d(a.U[e], a.U[ne + e], a.U[2 * ne + e], a.eos);
+        double s = __builtin_fabs(w.u) + sound_speed(w.r, w.p, a.eos);
+        sm = (unsigned long long)__double_as_longlong(s);
+    }
+    sm = wave_max_u64(sm);
+    if ((threadIdx.x & 63) == 0) atomicMax(&a.ps->smax[0], sm);
+}
+
+// periodic (benchmark_euler1d.cpp:102-110) or outflow ghost fill, 3 variables
+__global__ void fill1d_kernel(double* U, int n, int bc, const nest_prob_state* ps) {
+    if (ps && ps->done) return;
+    const int t = threadIdx.x;  // 3 vars x 2 ghosts x 2 sides = 12 threads
+    if (t >= 12) return;
+    const int ne = n + 4;
+    const int k = t / 4, g = (t / 2) & 1, side = t & 1;
+    double* u = U + k * ne;
+    if (bc == 0) {
+        if (side == 0) u[g] = u[n + g];
+        else u[2 + n + g] = u[2 + g];
+    } else {
+        if (side == 0) u[g] = u[2];
+        else u[2 + n + g] = u[n + 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <int STAGE>
+static void stage2d_dispatch(int lim, const stage_args& a, int grid, int block, hipStream_t s) {
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((stage2d_kernel<LIM_MC, STAGE>), grid, block, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((stage2d_kernel<LIM_NONE, STAGE>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((stage2d_kernel<LIM_MINMOD, STAGE>), grid, block, 0, s, a); break;
+    }
+}
+
+void launch_stage2d(int stage, int lim, const stage_args& a, int grid, int block, hipStream_t s) {
+    if (stage == 1) stage2d_dispatch<1>(lim, a, grid, block, s);
+    else stage2d_dispatch<2>(lim, a, grid, block, s);
+}
+
+void launch_speeds2d(const stage_args& a, int grid, int block, hipStream_t s) {
+    hipLaunchKernelGGL(speeds2d_kernel, grid, block, 0, s, a);
+}
+
+void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars, hipStream_t s) {
+    if (nreg <= 0) return;
+    hipLaunchKernelGGL(fill_regions_kernel, nreg, 256, 0, s, base, R, nreg, nvars);
+}
+
+void launch_finalize(nest_prob_state* P, int nprob, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, (nprob + 255) / 256, 256, 0, s, P, nprob);
+}
+
+void launch_reset_speeds(nest_prob_state* P, int nprob, hipStream_t s) {
+    hipLaunchKernelGGL(reset_speeds_kernel, (nprob + 255) / 256, 256, 0, s, P, nprob);
+}
+
+template <int STAGE>
+static void stage1d_dispatch(int lim, const stage1d_args& a, int grid, hipStream_t s) {
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((stage1d_kernel<LIM_MC, STAGE>), grid, 256, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((stage1d_kernel<LIM_NONE, STAGE>), grid, 256, 0, s, a); break;
+        default: hipLaunchKernelGGL((stage1d_kernel<LIM_MINMOD, STAGE>), grid, 256, 0, s, a); break;
+    }
+}
+
+void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s) {
+    const int grid = (a.n + 255) / 256;
+    if (stage == 1) stage1d_dispatch<1>(lim, a, grid, s);
+    else stage1d_dispatch<2>(lim, a, grid, s);
+}
+
+void launch_speeds1d(const stage1d_args& a, hipStream_t s) {
+    hipLaunchKernelGGL(speeds1d_kernel, (a.n + 255) / 256, 256, 0, s, a);
+}
+
+void launch_fill1d(double* U, int n, int bc, const nest_prob_state* ps, hipStream_t s) {
+    hipLaunchKernelGGL(fill1d_kernel, 1, 64, 0, s, U, n, bc, ps);
+}
+
+}  // namespace NEST_NS

@@ -1,0 +1,45 @@
+// Launch interface of the forward kernels (kernels.hip). The file is compiled
+// once per numerics mode; each compilation defines the same launchers inside
+// its own namespace (nest_exact / nest_fast), selected at run time by the
+// host runtime (nest_hip.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nest_types.hpp"
+
+#define NEST_DECLARE_LAUNCHERS(NS)                                                                  \
+    namespace NS {                                                                                  \
+    void launch_stage2d(int stage, int lim, const stage_args& a, int grid, int block, hipStream_t s); \
+    void launch_speeds2d(const stage_args& a, int grid, int block, hipStream_t s);                  \
+    void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars, hipStream_t s); \
+    void launch_finalize(nest_prob_state* P, int nprob, hipStream_t s);                             \
+    void launch_reset_speeds(nest_prob_state* P, int nprob, hipStream_t s);                         \
+    void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s);                  \
+    void launch_speeds1d(const stage1d_args& a, hipStream_t s);                                     \
+    void launch_fill1d(double* U, int n, int bc, const nest_prob_state* ps, hipStream_t s);         \
+    }
+
+// arguments of the 2D stage / speed kernels
+struct stage_args {
+    double* U;   // U^n, updated in place to U^{n+1} by stage 2
+    double* U1;  // first-stage state
+    const nest_patch_desc* patches;
+    const nest_item* items;
+    int nitems;
+    nest_prob_state* probs;
+    nest_eos eos;
+    double host_dt;  // > 0: use this dt for every problem (ignores done flags)
+};
+
+// arguments of the 1D kernels (one problem, one patch)
+struct stage1d_args {
+    double* U;
+    double* U1;
+    int n;
+    nest_prob_state* ps;
+    nest_eos eos;
+    double host_dt;  // > 0: use this dt (ignores the done flag)
+};
+
+NEST_DECLARE_LAUNCHERS(nest_exact)
+NEST_DECLARE_LAUNCHERS(nest_fast)

@@ -1,0 +1,229 @@
+"""Python front end of the gfx950 Euler backend (over the C ABI).
+
+Mirrors the reference's host objects for this path:
+  * ``Solver1D``  — the CUDA 1D solver object behind
+    src/backend_cuda/euler1d_solver.hpp:27-87 (create / upload / download /
+    compute_dt / rk2_step / run);
+  * ``Sim2D``     — the 2D patch-mesh driver loop of
+    examples/sod2d/sod2d.cpp:318-356 (exchange, x_sweep_t, exchange,
+    y_sweep_t; euler2d.hpp:67-239) with compute_dt_2d (euler2d.hpp:242-271),
+    stepped as SSP-RK2 over the X-then-Y split stage; ``Sim2D.batch`` holds
+    many independent problems stepped together.
+
+Arrays are float64 numpy arrays in field_t layout (variable-major, last
+index contiguous) over each patch's extended space (halo 2).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BC, LIMITER, MODE, check
+
+_pd = C.POINTER(C.c_double)
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_pd)
+
+
+class Solver1D:
+    """Single 1D patch, n interior cells, halo 2, 3 conserved variables."""
+
+    def __init__(self, n: int, dx: float, gamma: float = 1.4, bc: str = "outflow", limiter: str = "minmod",
+                 mode: str = "exact"):
+        self.L = _lib.load()
+        self.n, self.dx = n, dx
+        h = C.c_void_p()
+        check(self.L.nest_hip_solver1d_create(n, dx, gamma, BC[bc], LIMITER[limiter], MODE[mode], C.byref(h)))
+        self.h = h
+
+    @property
+    def size(self) -> int:
+        return 3 * (self.n + 4)
+
+    def upload(self, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        assert U.size == self.size
+        check(self.L.nest_hip_solver1d_upload(self.h, _dptr(U)))
+
+    def download(self) -> np.ndarray:
+        U = np.empty(self.size)
+        check(self.L.nest_hip_solver1d_download(self.h, _dptr(U)))
+        return U
+
+    def compute_dt(self, cfl: float) -> float:
+        dt = C.c_double()
+        check(self.L.nest_hip_solver1d_compute_dt(self.h, cfl, C.byref(dt)))
+        return dt.value
+
+    def rk2_step(self, dt: float) -> None:
+        check(self.L.nest_hip_solver1d_rk2_step(self.h, dt))
+
+    def run(self, t_final: float, cfl: float) -> int:
+        k = C.c_int()
+        check(self.L.nest_hip_solver1d_run(self.h, t_final, cfl, C.byref(k)))
+        return k.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nest_hip_solver1d_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class Sim2D:
+    """A 2D patch mesh (one problem) or a batch of independent problems."""
+
+    def __init__(self, handle: C.c_void_p):
+        self.L = _lib.load()
+        self.h = handle
+        total, npatch, nprob, nitems = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
+        check(self.L.nest_hip_sim2d_info(self.h, C.byref(total), C.byref(npatch), C.byref(nprob), C.byref(nitems)))
+        self.total, self.npatch, self.nprob, self.nitems = total.value, npatch.value, nprob.value, nitems.value
+        self.patches = []
+        for p in range(self.npatch):
+            off, nx, ny, prob = C.c_longlong(), C.c_int(), C.c_int(), C.c_int()
+            check(self.L.nest_hip_sim2d_patch_info(self.h, p, C.byref(off), C.byref(nx), C.byref(ny),
+                                                   C.byref(prob)))
+            self.patches.append((off.value, nx.value, ny.value, prob.value))
+
+    @classmethod
+    def mesh(cls, nx, ny, npx=1, npy=1, dx=None, dy=None, gamma=1.4, bcx="outflow", bcy="outflow",
+             limiter="minmod", mode="exact"):
+        L = _lib.load()
+        dx = 1.0 / nx if dx is None else dx
+        dy = 1.0 / ny if dy is None else dy
+        h = C.c_void_p()
+        check(L.nest_hip_mesh2d_create(nx, ny, npx, npy, dx, dy, gamma, BC[bcx], BC[bcy], LIMITER[limiter],
+                                       MODE[mode], C.byref(h)))
+        sim = cls(h)
+        sim.grid = (nx, ny, npx, npy)
+        return sim
+
+    @classmethod
+    def batch(cls, sizes, dx=None, dy=None, gamma=1.4, bc="outflow", limiter="minmod", mode="exact"):
+        L = _lib.load()
+        n = len(sizes)
+        nx = (C.c_int * n)(*[s[0] for s in sizes])
+        ny = (C.c_int * n)(*[s[1] for s in sizes])
+        dxs = (C.c_double * n)(*(dx if dx is not None else [1.0 / s[0] for s in sizes]))
+        dys = (C.c_double * n)(*(dy if dy is not None else [1.0 / s[1] for s in sizes]))
+        h = C.c_void_p()
+        check(L.nest_hip_batch2d_create(n, nx, ny, dxs, dys, gamma, BC[bc], LIMITER[limiter], MODE[mode],
+                                        C.byref(h)))
+        return cls(h)
+
+    # ---- data movement ----
+    def upload(self, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        assert U.size == self.total, (U.size, self.total)
+        check(self.L.nest_hip_sim2d_upload(self.h, _dptr(U)))
+
+    def download(self) -> np.ndarray:
+        U = np.empty(self.total)
+        check(self.L.nest_hip_sim2d_download(self.h, _dptr(U)))
+        return U
+
+    def patch_view(self, U: np.ndarray, p: int) -> np.ndarray:
+        off, nx, ny, _ = self.patches[p]
+        return U[off: off + 4 * (nx + 4) * (ny + 4)].reshape(4, nx + 4, ny + 4)
+
+    def _origins(self):
+        """(x0, y0) of every patch of a mesh (subspace() partition)."""
+        nx, ny, npx, npy = self.grid
+
+        def starts(total, n):
+            base, extra = divmod(total, n)
+            return [p * base + min(p, extra) for p in range(n)]
+
+        xs, ys = starts(nx, npx), starts(ny, npy)
+        return [(xs[p % npx], ys[p // npx]) for p in range(self.npatch)]
+
+    def scatter(self, G: np.ndarray) -> np.ndarray:
+        """Global interior array G[4, nx, ny] -> concatenated patch buffer (halos zero)."""
+        U = np.zeros(self.total)
+        for p, (x0, y0) in enumerate(self._origins()):
+            _, lx, ly, _ = self.patches[p]
+            self.patch_view(U, p)[:, 2:2 + lx, 2:2 + ly] = G[:, x0:x0 + lx, y0:y0 + ly]
+        return U
+
+    def gather(self, U: np.ndarray) -> np.ndarray:
+        """Concatenated patch buffer -> global interior array G[4, nx, ny]."""
+        nx, ny, _, _ = self.grid
+        G = np.empty((4, nx, ny))
+        for p, (x0, y0) in enumerate(self._origins()):
+            _, lx, ly, _ = self.patches[p]
+            G[:, x0:x0 + lx, y0:y0 + ly] = self.patch_view(U, p)[:, 2:2 + lx, 2:2 + ly]
+        return G
+
+    def device_ptrs(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        check(self.L.nest_hip_sim2d_device_ptrs(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def bind(self, dev_U: int | None, dev_U1: int | None = None) -> None:
+        check(self.L.nest_hip_sim2d_bind(self.h, C.c_void_p(dev_U or 0), C.c_void_p(dev_U1 or 0)))
+
+    def set_stream(self, stream: int | None) -> None:
+        check(self.L.nest_hip_sim2d_set_stream(self.h, C.c_void_p(stream or 0)))
+
+    def set_tiling(self, rows=0, waves_per_block=0, persistent=0) -> None:
+        check(self.L.nest_hip_sim2d_set_tiling(self.h, rows, waves_per_block, persistent))
+        total, npatch, nprob, nitems = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
+        check(self.L.nest_hip_sim2d_info(self.h, C.byref(total), C.byref(npatch), C.byref(nprob), C.byref(nitems)))
+        self.nitems = nitems.value
+
+    # ---- numerics ----
+    def fill_halos(self) -> None:
+        check(self.L.nest_hip_sim2d_fill_halos(self.h))
+
+    def compute_dt(self, cfl: float) -> np.ndarray:
+        dt = np.zeros(self.nprob)
+        check(self.L.nest_hip_sim2d_compute_dt(self.h, cfl, _dptr(dt)))
+        return dt
+
+    def stage(self, stage: int, dt: float, patch: int = -1) -> None:
+        check(self.L.nest_hip_sim2d_stage(self.h, stage, dt, patch))
+
+    def rk2_step(self, dt: float) -> None:
+        check(self.L.nest_hip_sim2d_rk2_step(self.h, dt))
+
+    def start(self, cfl: float = 0.4, t_end: float = float("inf"), fixed_dt: float = 0.0) -> None:
+        check(self.L.nest_hip_sim2d_start(self.h, cfl, t_end, fixed_dt))
+
+    def advance(self, nsteps: int, graph: bool = False) -> None:
+        check(self.L.nest_hip_sim2d_advance(self.h, nsteps, int(graph)))
+
+    def advance_per_patch(self, nsteps: int, cfl: float) -> None:
+        check(self.L.nest_hip_sim2d_advance_per_patch(self.h, nsteps, cfl))
+
+    def status(self):
+        t, dt = np.zeros(self.nprob), np.zeros(self.nprob)
+        steps, done = np.zeros(self.nprob, dtype=np.int32), np.zeros(self.nprob, dtype=np.int32)
+        check(self.L.nest_hip_sim2d_status(self.h, _dptr(t), _dptr(dt),
+                                           steps.ctypes.data_as(C.POINTER(C.c_int)),
+                                           done.ctypes.data_as(C.POINTER(C.c_int))))
+        return {"t": t, "dt": dt, "steps": steps, "done": done}
+
+    def advance_timed(self, nsteps: int):
+        """nsteps device-dt steps with HIP events around each launch; returns
+        (per-component ms [finalize, fill U, stage 1, fill U1, stage 2], total ms)."""
+        ms = np.zeros(5)
+        tot = C.c_double()
+        check(self.L.nest_hip_sim2d_advance_timed(self.h, nsteps, _dptr(ms), C.byref(tot)))
+        return ms, tot.value
+
+    def synchronize(self) -> None:
+        check(self.L.nest_hip_sim2d_synchronize(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nest_hip_sim2d_destroy(self.h)
+            self.h = None
+
+    __del__ = close

@@ -1,0 +1,265 @@
+"""GPU parity: HIP kernels (through the C ABI) against the CPU oracle.
+
+Exact mode must be bitwise equal to the oracle (IEEE division and sqrt, no
+FMA contraction on either side). Fast mode must agree to 1e-12 relative
+L-infinity per step; measured deviations are ~1e-15.
+"""
+import numpy as np
+import pytest
+
+from nest_amd import Sim2D, Solver1D, problems as P
+from oracle.oracle import MeshSpec
+from tests.gpu_util import print_case, rel_linf_vars
+
+pytestmark = pytest.mark.gpu
+TOL_FAST = 1e-12
+
+
+# ----------------------------------------------------------------------------
+# 1D
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("bc", ["outflow", "periodic"])
+def test_1d_sod_lockstep(oracle, mode, bc):
+    """Config 1 (Sod N=1000, CFL 0.4): same U^n and dt fed to both each step."""
+    n, cfl = 1000, 0.4
+    dx = 1.0 / n
+    print_case("1d lockstep", mode, bc)
+    U = P.sod1d(n)
+    oracle.fill_1d(U, n, bc=bc)
+    s = Solver1D(n, dx, bc=bc, mode=mode)
+    worst = 0.0
+    for k in range(40):
+        s.upload(U)
+        dt_gpu = s.compute_dt(cfl)
+        dt_cpu = cfl * dx / oracle.max_speed_1d(U, n)
+        if mode == "exact":
+            assert dt_gpu == dt_cpu
+        else:
+            assert abs(dt_gpu - dt_cpu) <= 1e-14 * dt_cpu
+        s.rk2_step(dt_cpu)
+        got = s.download().reshape(3, -1)[:, 2:-2]
+        ref = U.copy()
+        oracle.rk2_step_1d(ref, n, dt_cpu, dx, bc=bc)
+        want = ref.reshape(3, -1)[:, 2:-2]
+        if mode == "exact":
+            assert np.array_equal(got, want), f"step {k}"
+        else:
+            worst = max(worst, rel_linf_vars(got, want))
+        U = ref
+    assert worst <= TOL_FAST
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_1d_sod_run_to_t_end(oracle, mode):
+    """Config 1 end to end with device-side dt: step count and final state."""
+    n, cfl, t_end = 1000, 0.4, 0.2
+    print_case("1d run", mode)
+    U = P.sod1d(n)
+    oracle.fill_1d(U, n, bc="outflow")
+    ref = U.copy()
+    k_ref, _ = oracle.run_1d(ref, n, t_end, cfl, 1.0 / n, bc="outflow")
+    s = Solver1D(n, 1.0 / n, bc="outflow", mode=mode)
+    s.upload(U)
+    k = s.run(t_end, cfl)
+    got = s.download().reshape(3, -1)[:, 2:-2]
+    want = ref.reshape(3, -1)[:, 2:-2]
+    assert k == k_ref
+    if mode == "exact":
+        assert np.array_equal(got, want)
+    else:
+        assert rel_linf_vars(got, want) <= 1e-10  # free-running over ~1000 steps
+    # conservation of mass with outflow: only boundary fluxes change it; the
+    # Sod state is quiescent at both ends through t = 0.2
+    assert abs(got[0].sum() - want[0].sum()) <= 1e-12 * want[0].sum()
+
+
+def test_1d_reference_abi_39_steps(oracle):
+    """Reference KAT (docs/CUDA_BACKEND.md:148-166, tests/test_euler_cuda.cpp:27-43):
+    periodic Sod N=100, t=0.1, CFL 0.3 takes 39 steps; GPU equals CPU."""
+    from nest_amd import _lib
+    import ctypes as C
+    L = _lib.load()
+    n = 100
+    U = P.sod1d(n)
+    h = L.euler1d_gpu_create(n, 2, 1.0 / n, 1.4)
+    assert h
+    try:
+        L.euler1d_gpu_upload(h, U.ctypes.data_as(C.POINTER(C.c_double)))
+        steps = L.euler1d_gpu_run(h, 0.1, 0.3)
+        out = np.empty_like(U)
+        L.euler1d_gpu_download(h, out.ctypes.data_as(C.POINTER(C.c_double)))
+    finally:
+        L.euler1d_gpu_destroy(h)
+    ref = U.copy()
+    oracle.fill_1d(ref, n, bc="periodic")
+    k_ref, _ = oracle.run_1d(ref, n, 0.1, 0.3, 1.0 / n, bc="periodic")
+    assert steps == 39 == k_ref
+    assert np.array_equal(out.reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2])
+
+
+# ----------------------------------------------------------------------------
+# 2D mesh
+# ----------------------------------------------------------------------------
+CASES_2D = [
+    # nx, ny, npx, npy, bcx, bcy, state
+    (96, 80, 3, 2, "outflow", "outflow", "sod"),
+    (64, 64, 2, 2, "periodic", "periodic", "quad"),
+    (70, 130, 2, 3, "periodic", "outflow", "quad"),
+    (40, 200, 1, 1, "outflow", "periodic", "sod"),
+]
+
+
+def _state(kind, nx, ny):
+    if kind == "sod":
+        return P.sod2d(nx, ny, jitter=1e-3, seed=2)
+    G = P.quadrants(max(nx, ny), 7)[:, :nx, :ny]
+    return np.ascontiguousarray(G)
+
+
+@pytest.mark.parametrize("case", CASES_2D)
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_2d_mesh_lockstep(oracle, case, mode):
+    nx, ny, npx, npy, bcx, bcy, kind = case
+    print_case("2d lockstep", case, mode)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    sim = Sim2D.mesh(nx, ny, npx, npy, dx, dy, bcx=bcx, bcy=bcy, mode=mode)
+    m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+    U = sim.scatter(_state(kind, nx, ny))
+    oracle.mesh_fill(U, m)
+    worst = 0.0
+    for k in range(6):
+        sim.upload(U)
+        dt_gpu = sim.compute_dt(0.4)[0]
+        dt_cpu = oracle.mesh_dt(U, m, dx, dy, 0.4)
+        if mode == "exact":
+            assert dt_gpu == dt_cpu
+        else:
+            assert abs(dt_gpu - dt_cpu) <= 1e-14 * dt_cpu
+        sim.rk2_step(dt_cpu)
+        got = sim.download()
+        ref = U.copy()
+        oracle.mesh_rk2_step(ref, m, dt_cpu, dx, dy)
+        if mode == "exact":
+            assert np.array_equal(got, ref), f"step {k}: max diff {np.abs(got - ref).max()}"
+        else:
+            worst = max(worst, rel_linf_vars(sim.gather(got), sim.gather(ref)))
+        U = ref
+    assert worst <= TOL_FAST
+
+
+def test_2d_halo_fill_bitwise(oracle):
+    """Device exchange plan (faces + corners, periodic and outflow) == oracle."""
+    for (nx, ny, npx, npy, bcx, bcy, _) in CASES_2D:
+        print_case("halo", nx, ny, npx, npy, bcx, bcy)
+        sim = Sim2D.mesh(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+        m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+        rng = np.random.default_rng(nx * ny)
+        U = sim.scatter(rng.uniform(-1, 1, (4, nx, ny)))
+        sim.upload(U)
+        sim.fill_halos()
+        ref = U.copy()
+        oracle.mesh_fill(ref, m)
+        assert np.array_equal(sim.download(), ref)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_2d_device_dt_run(oracle, graph):
+    """Device-resident dt (reduction fused into stage 2 + finalize kernel):
+    dt sequence and states equal the oracle's compute_dt_2d loop."""
+    nx, ny, npx, npy = 64, 48, 2, 2
+    print_case("2d device dt", graph)
+    sim = Sim2D.mesh(nx, ny, npx, npy)
+    m = MeshSpec(nx, ny, npx, npy)
+    U = sim.scatter(P.sod2d(nx, ny))
+    oracle.mesh_fill(U, m)
+    sim.upload(U)
+    sim.start(cfl=0.4, t_end=0.05)
+    ref = U.copy()
+    k_ref, dts = oracle.mesh_run(ref, m, 1000, 1.0 / nx, 1.0 / ny, cfl=0.4, t_end=0.05)
+    sim.advance(k_ref + 3, graph=graph)  # extra steps must be no-ops once t_end is reached
+    st = sim.status()
+    assert st["steps"][0] == k_ref and st["done"][0] == 1
+    assert st["t"][0] == np.cumsum(dts)[-1] or abs(st["t"][0] - 0.05) < 1e-15
+    assert np.array_equal(sim.gather(sim.download()), sim.gather(ref))
+
+
+def test_2d_limiters(oracle):
+    nx, ny = 48, 40
+    for lim in ["mc", "none"]:
+        for mode in ["exact", "fast"]:
+            print_case("limiter", lim, mode)
+            sim = Sim2D.mesh(nx, ny, 2, 2, limiter=lim, mode=mode, bcx="periodic", bcy="periodic")
+            m = MeshSpec(nx, ny, 2, 2, bcx="periodic", bcy="periodic")
+            U = sim.scatter(P.smooth_periodic(48)[:, :nx, :ny].copy())
+            oracle.mesh_fill(U, m)
+            dt = oracle.mesh_dt(U, m, 1 / nx, 1 / ny, 0.4)
+            sim.upload(U)
+            sim.rk2_step(dt)
+            ref = U.copy()
+            oracle.mesh_rk2_step(ref, m, dt, 1 / nx, 1 / ny, lim=lim)
+            got = sim.download()
+            if mode == "exact":
+                assert np.array_equal(got, ref)
+            else:
+                assert rel_linf_vars(sim.gather(got), sim.gather(ref)) <= TOL_FAST
+
+
+def test_batch_matches_single_problems(oracle):
+    """Ensemble of uneven problems stepped together == each problem alone."""
+    sizes = [64, 81, 100, 130, 77]
+    print_case("batch", sizes)
+    sim = Sim2D.batch([(s, s) for s in sizes])
+    U = np.zeros(sim.total)
+    refs = []
+    for k, s in enumerate(sizes):
+        m = MeshSpec(s, s, 1, 1)
+        G = P.quadrants(s, k)
+        u = m.to_patches(G)
+        oracle.mesh_fill(u, m)
+        sim.patch_view(U, k)[:] = u.reshape(4, s + 4, s + 4)
+        refs.append((m, u))
+    sim.upload(U)
+    sim.start(cfl=0.4)
+    sim.advance(5, graph=True)
+    got = sim.download()
+    st = sim.status()
+    for k, (m, u) in enumerate(refs):
+        kk, dts = oracle.mesh_run(u, m, 5, 1.0 / m.nx, 1.0 / m.ny, cfl=0.4)
+        assert st["steps"][k] == 5
+        assert np.array_equal(sim.patch_view(got, k)[:, 2:-2, 2:-2], u.reshape(4, m.nx + 4, m.ny + 4)[:, 2:-2, 2:-2])
+
+
+def test_per_patch_path_equals_batched(oracle):
+    nx, ny, npx, npy = 128, 96, 4, 3
+    print_case("per-patch vs batched")
+    a = Sim2D.mesh(nx, ny, npx, npy)
+    b = Sim2D.mesh(nx, ny, npx, npy)
+    U = a.scatter(P.sod2d(nx, ny))
+    m = MeshSpec(nx, ny, npx, npy)
+    oracle.mesh_fill(U, m)
+    a.upload(U)
+    b.upload(U)
+    a.advance_per_patch(4, 0.4)
+    b.start(cfl=0.4)
+    b.advance(4)
+    ga, gb = a.gather(a.download()), b.gather(b.download())
+    assert np.array_equal(ga, gb)
+
+
+def test_config2_full_size_one_step(oracle):
+    """Config 2 at full size (4096^2, 16x16 patches of 256^2): one RK2 step
+    bitwise vs the oracle, plus the seeded jitter's mass total."""
+    n, npt = 4096, 16
+    print_case("config2 full size")
+    sim = Sim2D.mesh(n, n, npt, npt)
+    m = MeshSpec(n, n, npt, npt)
+    U = sim.scatter(P.sod2d(n, n))
+    oracle.mesh_fill(U, m)
+    dt = oracle.mesh_dt(U, m, 1.0 / n, 1.0 / n, 0.4)
+    sim.upload(U)
+    assert sim.compute_dt(0.4)[0] == dt
+    sim.rk2_step(dt)
+    got = sim.download()
+    oracle.mesh_rk2_step(U, m, dt, 1.0 / n, 1.0 / n)
+    assert np.array_equal(got, U)
