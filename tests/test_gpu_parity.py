@@ -74,9 +74,12 @@ def test_1d_sod_run_to_t_end(oracle, mode):
     assert abs(got[0].sum() - want[0].sum()) <= 1e-12 * want[0].sum()
 
 
-def test_1d_reference_abi_39_steps(oracle):
-    """Reference KAT (docs/CUDA_BACKEND.md:148-166, tests/test_euler_cuda.cpp:27-43):
-    periodic Sod N=100, t=0.1, CFL 0.3 takes 39 steps; GPU equals CPU."""
+def test_1d_reference_abi_periodic_sod(oracle):
+    """Reference GPU-vs-CPU case (tests/test_euler_cuda.cpp:27-43): periodic
+    Sod N=100, t=0.1, CFL 0.3 through the reference-compatible euler1d_gpu_*
+    ABI. The reference's CPU loop (test_euler_cuda.cpp:82-106) takes 70 steps
+    (the oracle, pinned by the reference benchmark's own step counts, agrees);
+    docs/CUDA_BACKEND.md:160 prints 39, which its code does not produce."""
     from nest_amd import _lib
     import ctypes as C
     L = _lib.load()
@@ -94,8 +97,28 @@ def test_1d_reference_abi_39_steps(oracle):
     ref = U.copy()
     oracle.fill_1d(ref, n, bc="periodic")
     k_ref, _ = oracle.run_1d(ref, n, 0.1, 0.3, 1.0 / n, bc="periodic")
-    assert steps == 39 == k_ref
+    assert steps == k_ref == 70
     assert np.array_equal(out.reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2])
+
+
+def test_reference_gpu_test_program_links_and_passes():
+    """The reference's own tests/test_euler_cuda.cpp, compiled from its source
+    (oracle/Makefile target `drop-in`) and linked against libnest_hip.so
+    through the euler1d_gpu_* ABI: its CPU half must accept our GPU result
+    (its tolerance, 1e-8) — the drop-in boundary exercised by the reference's
+    own harness."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref", "test_euler_cuda")
+    if not os.path.exists(exe):
+        pytest.skip("reference test program not built (needs the reference tree at build time)")
+    print_case("reference test_euler_cuda")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS" in r.stdout and "GPU: 70 steps" in r.stdout
+    diffs = [float(l.split(":")[1]) for l in r.stdout.splitlines() if l.startswith("Max difference")]
+    assert len(diffs) == 3 and max(diffs) == 0.0  # bitwise equal to the reference CPU code
 
 
 # ----------------------------------------------------------------------------

@@ -1,0 +1,66 @@
+"""The C-ABI library loads without a GPU and exports every symbol that
+include/nest_hip.h declares (no compute calls here)."""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+from nest_amd import _lib
+
+
+def test_header_declares_expected_entry_points():
+    syms = _lib.header_symbols()
+    for s in ("nest_hip_mesh2d_create", "nest_hip_sim2d_advance", "euler1d_gpu_create", "euler1d_gpu_run_timed"):
+        assert s in syms
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    for s in _lib.header_symbols():
+        assert hasattr(L, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in _lib.header_symbols() if s not in exported]
+    assert not missing, missing
+
+
+def test_library_is_gfx950_code_object():
+    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o",
+                          f"--input={_lib.LIB_PATH}"], capture_output=True, text=True)
+    if out.returncode != 0:
+        pytest.skip("offload bundler cannot list a shared object on this host")
+    assert "gfx950" in out.stdout
+
+
+def test_error_reporting_without_device():
+    """Argument validation happens before any device call."""
+    L = _lib.load()
+    h = C.c_void_p()
+    rc = L.nest_hip_solver1d_create(1, 0.1, 1.4, 0, 0, 0, C.byref(h))
+    assert rc == 1 and "n >= 2" in _lib.last_error()
+    rc = L.nest_hip_mesh2d_create(8, 8, 8, 1, 0.1, 0.1, 1.4, 0, 0, 0, 0, C.byref(h))
+    assert rc == 1 and "2 interior cells" in _lib.last_error()
+    assert L.nest_hip_sim2d_info(None, None, None, None, None) == 1
+
+
+def test_no_cpu_fallback_when_library_missing(tmp_path):
+    with pytest.raises(_lib.NestLibraryMissing):
+        saved = _lib._lib
+        _lib._lib = None
+        try:
+            _lib.load(str(tmp_path / "absent.so"))
+        finally:
+            _lib._lib = saved
+
+
+def test_problems_are_decomposition_independent():
+    from nest_amd import problems as P
+    import numpy as np
+    a = P.uniform01(2, np.arange(100, dtype=np.uint64))
+    b = np.concatenate([P.uniform01(2, np.arange(0, 37, dtype=np.uint64)),
+                        P.uniform01(2, np.arange(37, 100, dtype=np.uint64))])
+    assert np.array_equal(a, b) and (a >= 0).all() and (a < 1).all()
+    sizes = P.ensemble_sizes()
+    assert len(sizes) == 512 and min(sizes) >= 64 and max(sizes) <= 512
