@@ -16,6 +16,7 @@
 
 #include "kernels.hpp"
 #include "nest_hip.h"
+#include "plan.hpp"
 
 namespace {
 
@@ -179,12 +180,10 @@ struct nest_sim2d {
         item_end.assign(patches_h.size(), 0);
         for (size_t p = 0; p < patches_h.size(); ++p) {
             item_begin[p] = (int)items_h.size();
-            const nest_patch_desc& d = patches_h[p];
-            for (int r0 = 0; r0 < d.nx; r0 += rows)
-                for (int c0 = 0; c0 < d.ny; c0 += 60)
-                    items_h.push_back({(int)p, c0, r0, std::min(r0 + rows, d.nx)});
+            nest_plan::add_items(items_h, (int)p, patches_h[p].nx, patches_h[p].ny, rows);
             item_end[p] = (int)items_h.size();
         }
+        if (int rc = validate_tables()) return rc;
         if (items_d) (void)hipFree(items_d);
         items_d = nullptr;
         if (int rc = dev_alloc(&items_d, items_h.size())) return rc;
@@ -193,7 +192,17 @@ struct nest_sim2d {
         return NEST_OK;
     }
 
+    // Every device table entry must address memory inside the buffers: checked
+    // on the host before anything is uploaded or launched.
+    int validate_tables() const {
+        if (!nest_plan::tables_in_bounds(patches_h, regions_h, items_h, (long long)total))
+            return fail(NEST_ERR_INVALID, "internal: a patch / halo region / work item addresses memory "
+                                          "outside the buffers");
+        return NEST_OK;
+    }
+
     int init_device() {
+        if (int rc = validate_tables()) return rc;
         if (int rc = dev_alloc(&U_own, total)) return rc;
         if (int rc = dev_alloc(&U1_own, total)) return rc;
         U = U_own;
@@ -240,82 +249,7 @@ struct nest_sim2d {
 
 namespace {
 
-// halo regions of one patch (px, py) of a tensor-product mesh: faces and
-// corners, each mapped through the per-dimension boundary condition.
-struct dim_part {
-    std::vector<int> start, len;
-};
-
-dim_part partition(int total, int n) {  // subspace() semantics, reference core.hpp:184-199
-    dim_part d;
-    const int base = total / n, extra = total % n;
-    for (int p = 0; p < n; ++p) {
-        d.start.push_back(p * base + std::min(p, extra));
-        d.len.push_back(base + (p < extra ? 1 : 0));
-    }
-    return d;
-}
-
-// source of the ghost strip on `side` (-1, 0, +1) of partition p along one
-// dimension: (source partition, first source extended index, broadcast)
-void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& s, int& bcast) {
-    const int np = (int)d.len.size();
-    const int h = 2;
-    bcast = 0;
-    if (side == 0) {
-        q = p;
-        s = h;
-    } else if (side < 0) {
-        if (p > 0 || bc == NEST_BC_PERIODIC) {
-            q = (p - 1 + np) % np;
-            s = d.len[q];  // = h + len - h: last h interior cells
-        } else {
-            q = p;
-            s = h;
-            bcast = 1;
-        }
-    } else {
-        if (p < np - 1 || bc == NEST_BC_PERIODIC) {
-            q = (p + 1) % np;
-            s = h;
-        } else {
-            q = p;
-            s = h + d.len[p] - 1;
-            bcast = 1;
-        }
-    }
-}
-
-void add_mesh_regions(std::vector<nest_region>& R, const dim_part& X, const dim_part& Y, int bcx, int bcy,
-                      const std::vector<long long>& off, int patch_base) {
-    const int npx = (int)X.len.size(), npy = (int)Y.len.size();
-    const int h = 2;
-    for (int py = 0; py < npy; ++py)
-        for (int px = 0; px < npx; ++px) {
-            const int p = patch_base + py * npx + px;
-            const int lx = X.len[px], ly = Y.len[py];
-            for (int sx = -1; sx <= 1; ++sx)
-                for (int sy = -1; sy <= 1; ++sy) {
-                    if (sx == 0 && sy == 0) continue;
-                    nest_region g{};
-                    int qx, qy;
-                    dim_source(sx, px, X, bcx, qx, g.si, g.bi);
-                    dim_source(sy, py, Y, bcy, qy, g.sj, g.bj);
-                    const int q = patch_base + qy * npx + qx;
-                    g.di = sx < 0 ? 0 : (sx == 0 ? h : h + lx);
-                    g.ni = sx == 0 ? lx : h;
-                    g.dj = sy < 0 ? 0 : (sy == 0 ? h : h + ly);
-                    g.nj = sy == 0 ? ly : h;
-                    g.dst_off = off[p];
-                    g.src_off = off[q];
-                    g.dst_nye = ly + 2 * h;
-                    g.src_nye = Y.len[qy] + 2 * h;
-                    g.dst_vs = (lx + 2 * h) * (ly + 2 * h);
-                    g.src_vs = (X.len[qx] + 2 * h) * (Y.len[qy] + 2 * h);
-                    R.push_back(g);
-                }
-        }
-}
+using namespace nest_plan;
 
 nest_prob_state make_prob(double dx, double dy, int dim) {
     nest_prob_state s{};
@@ -360,15 +294,9 @@ int nest_hip_mesh2d_create(int nx, int ny, int npx, int npy, double dx, double d
     s->eos = make_eos(gamma);
     s->nprob = 1;
     s->probs_h.push_back(make_prob(dx, dy, 2));
-    dim_part X = partition(nx, npx), Y = partition(ny, npy);
-    std::vector<long long> off;
-    for (int py = 0; py < npy; ++py)
-        for (int px = 0; px < npx; ++px) {
-            off.push_back((long long)s->total);
-            s->patches_h.push_back({(long long)s->total, X.len[px], Y.len[py], 0, 0});
-            s->total += 4 * (size_t)(X.len[px] + 4) * (Y.len[py] + 4);
-        }
-    add_mesh_regions(s->regions_h, X, Y, bcx, bcy, off, 0);
+    long long total = 0;
+    build_mesh(nx, ny, npx, npy, bcx, bcy, 0, s->patches_h, s->regions_h, total);
+    s->total = (size_t)total;
     if (int rc = s->init_device()) {
         delete s;
         return rc;
@@ -389,18 +317,16 @@ int nest_hip_batch2d_create(int nprob, const int* nx, const int* ny, const doubl
     s->lim = limiter;
     s->eos = make_eos(gamma);
     s->nprob = nprob;
+    long long total = 0;
     for (int i = 0; i < nprob; ++i) {
         if (nx[i] < 2 || ny[i] < 2 || !(dx[i] > 0.0) || !(dy[i] > 0.0)) {
             delete s;
             return fail(NEST_ERR_INVALID, "problem " + std::to_string(i) + ": bad size or spacing");
         }
         s->probs_h.push_back(make_prob(dx[i], dy[i], 2));
-        std::vector<long long> off{(long long)s->total};
-        s->patches_h.push_back({(long long)s->total, nx[i], ny[i], i, 0});
-        dim_part X = partition(nx[i], 1), Y = partition(ny[i], 1);
-        add_mesh_regions(s->regions_h, X, Y, bc, bc, off, i);
-        s->total += 4 * (size_t)(nx[i] + 4) * (ny[i] + 4);
+        build_mesh(nx[i], ny[i], 1, 1, bc, bc, i, s->patches_h, s->regions_h, total);
     }
+    s->total = (size_t)total;
     if (int rc = s->init_device()) {
         delete s;
         return rc;

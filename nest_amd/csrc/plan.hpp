@@ -1,0 +1,153 @@
+// Host-side construction of the device tables of a 2D problem set: the
+// tensor-product patch partition, the halo exchange plan (faces + corners,
+// periodic / outflow / patch-to-patch) and the stage work list. Pure C++ (no
+// HIP), so the exact tables the GPU consumes are also checked on the CPU
+// (tests/cpp/test_plan.cpp executes them against the oracle's halo fill).
+#pragma once
+#include <algorithm>
+#include <vector>
+
+#include "nest_types.hpp"
+
+namespace nest_plan {
+
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
+
+// halo regions of one patch (px, py) of a tensor-product mesh: faces and
+// corners, each mapped through the per-dimension boundary condition.
+struct dim_part {
+    std::vector<int> start, len;
+};
+
+inline dim_part partition(int total, int n) {  // subspace() semantics, reference core.hpp:184-199
+    dim_part d;
+    const int base = total / n, extra = total % n;
+    for (int p = 0; p < n; ++p) {
+        d.start.push_back(p * base + std::min(p, extra));
+        d.len.push_back(base + (p < extra ? 1 : 0));
+    }
+    return d;
+}
+
+// source of the ghost strip on `side` (-1, 0, +1) of partition p along one
+// dimension: (source partition, first source extended index, broadcast)
+inline void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& s, int& bcast) {
+    const int np = (int)d.len.size();
+    const int h = 2;
+    bcast = 0;
+    if (side == 0) {
+        q = p;
+        s = h;
+    } else if (side < 0) {
+        if (p > 0 || bc == BC_PERIODIC) {
+            q = (p - 1 + np) % np;
+            s = d.len[q];  // = h + len - h: last h interior cells
+        } else {
+            q = p;
+            s = h;
+            bcast = 1;
+        }
+    } else {
+        if (p < np - 1 || bc == BC_PERIODIC) {
+            q = (p + 1) % np;
+            s = h;
+        } else {
+            q = p;
+            s = h + d.len[p] - 1;
+            bcast = 1;
+        }
+    }
+}
+
+// off[] holds the array offsets of this mesh's patches in local patch-index
+// order (py*npx + px).
+inline void add_mesh_regions(std::vector<nest_region>& R, const dim_part& X, const dim_part& Y, int bcx, int bcy,
+                      const std::vector<long long>& off) {
+    const int npx = (int)X.len.size(), npy = (int)Y.len.size();
+    const int h = 2;
+    for (int py = 0; py < npy; ++py)
+        for (int px = 0; px < npx; ++px) {
+            const int p = py * npx + px;
+            const int lx = X.len[px], ly = Y.len[py];
+            for (int sx = -1; sx <= 1; ++sx)
+                for (int sy = -1; sy <= 1; ++sy) {
+                    if (sx == 0 && sy == 0) continue;
+                    nest_region g{};
+                    int qx, qy;
+                    dim_source(sx, px, X, bcx, qx, g.si, g.bi);
+                    dim_source(sy, py, Y, bcy, qy, g.sj, g.bj);
+                    const int q = qy * npx + qx;
+                    g.di = sx < 0 ? 0 : (sx == 0 ? h : h + lx);
+                    g.ni = sx == 0 ? lx : h;
+                    g.dj = sy < 0 ? 0 : (sy == 0 ? h : h + ly);
+                    g.nj = sy == 0 ? ly : h;
+                    g.dst_off = off[p];
+                    g.src_off = off[q];
+                    g.dst_nye = ly + 2 * h;
+                    g.src_nye = Y.len[qy] + 2 * h;
+                    g.dst_vs = (lx + 2 * h) * (ly + 2 * h);
+                    g.src_vs = (X.len[qx] + 2 * h) * (Y.len[qy] + 2 * h);
+                    R.push_back(g);
+                }
+        }
+}
+
+// Append one problem's patch mesh (nx*ny cells on npx*npy patches) to the
+// problem-set tables: patch descriptors at consecutive buffer offsets from
+// `total` (advanced), and its halo regions.
+inline void build_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int prob,
+                       std::vector<nest_patch_desc>& P, std::vector<nest_region>& R, long long& total) {
+    dim_part X = partition(nx, npx), Y = partition(ny, npy);
+    std::vector<long long> off;
+    for (int py = 0; py < npy; ++py)
+        for (int px = 0; px < npx; ++px) {
+            off.push_back(total);
+            P.push_back({total, X.len[px], Y.len[py], prob, 0});
+            total += 4LL * (X.len[px] + 4) * (Y.len[py] + 4);
+        }
+    add_mesh_regions(R, X, Y, bcx, bcy, off);
+}
+
+// Work list of one patch: row chunks of `rows`, strips of 60 output columns
+// (64-lane strips over the extended columns, 2 halo lanes each side).
+inline void add_items(std::vector<nest_item>& items, int patch, int nx, int ny, int rows) {
+    for (int r0 = 0; r0 < nx; r0 += rows)
+        for (int c0 = 0; c0 < ny; c0 += 60) items.push_back({patch, c0, r0, std::min(r0 + rows, nx)});
+}
+
+// Every table entry addresses memory inside a buffer of `total` doubles.
+inline bool tables_in_bounds(const std::vector<nest_patch_desc>& P, const std::vector<nest_region>& R,
+                             const std::vector<nest_item>& I, long long total) {
+    auto arr_ok = [&](long long off, long long vs) { return off >= 0 && off + 4 * vs <= total; };
+    for (const auto& d : P)
+        if (d.nx < 2 || d.ny < 2 || !arr_ok(d.off, (long long)(d.nx + 4) * (d.ny + 4))) return false;
+    for (const auto& g : R) {
+        const int dnx = g.dst_vs / g.dst_nye, snx = g.src_vs / g.src_nye;
+        bool ok = arr_ok(g.dst_off, g.dst_vs) && arr_ok(g.src_off, g.src_vs) && g.ni > 0 && g.nj > 0 &&
+                  g.di >= 0 && g.di + g.ni <= dnx && g.dj >= 0 && g.dj + g.nj <= g.dst_nye && g.si >= 0 &&
+                  g.si + (g.bi ? 1 : g.ni) <= snx && g.sj >= 0 && g.sj + (g.bj ? 1 : g.nj) <= g.src_nye;
+        if (!ok) return false;
+    }
+    for (const auto& it : I) {
+        if (it.patch < 0 || it.patch >= (int)P.size()) return false;
+        const auto& d = P[it.patch];
+        if (it.r0 < 0 || it.r0 >= it.r1 || it.r1 > d.nx || it.c0 < 0 || it.c0 >= d.ny) return false;
+    }
+    return true;
+}
+
+// Host execution of a region table (the same index arithmetic as the device
+// fill kernel), for CPU checks of the plan.
+inline void execute_regions_host(double* base, const std::vector<nest_region>& R, int nvars) {
+    for (const auto& g : R)
+        for (int k = 0; k < nvars; ++k)
+            for (int ia = 0; ia < g.ni; ++ia)
+                for (int jb = 0; jb < g.nj; ++jb) {
+                    const int si = g.si + (g.bi ? 0 : ia);
+                    const int sj = g.sj + (g.bj ? 0 : jb);
+                    base[g.dst_off + (long long)k * g.dst_vs + (long long)(g.di + ia) * g.dst_nye + g.dj + jb] =
+                        base[g.src_off + (long long)k * g.src_vs + (long long)si * g.src_nye + sj];
+                }
+}
+
+}  // namespace nest_plan

@@ -11,10 +11,20 @@ def print_case(*args):
 
 
 def rel_linf_vars(a, b):
-    """max over variables of max|a-b|/max|b| for arrays shaped [4, ...] or [3, ...]."""
+    """Relative L-infinity per conserved variable, max over variables:
+    max|a_k - b_k| / scale_k, arrays shaped [nvars, ...] (rho, momenta..., E).
+    scale_k = max|b_k|, except that a momentum's scale is floored at the
+    flow's momentum scale sqrt(max rho * max E) (~ rho*c): a momentum that is
+    identically zero or orders of magnitude below rho*c (the transverse
+    momentum of a planar problem) is otherwise measured against its own
+    round-off."""
     out = 0.0
-    for k in range(a.shape[0]):
+    nv = a.shape[0]
+    floor = np.sqrt(np.max(np.abs(b[0])) * np.max(np.abs(b[nv - 1])))
+    for k in range(nv):
         den = np.max(np.abs(b[k]))
+        if 0 < k < nv - 1:
+            den = max(den, floor)
         num = np.max(np.abs(a[k] - b[k]))
         out = max(out, num / den if den > 0 else num)
     return out

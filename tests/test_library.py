@@ -27,11 +27,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_library_is_gfx950_code_object():
-    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o",
-                          f"--input={_lib.LIB_PATH}"], capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("offload bundler cannot list a shared object on this host")
-    assert "gfx950" in out.stdout
+    """The embedded offload bundle targets gfx950 (our own build's fatbin)."""
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
 
 
 def test_error_reporting_without_device():
