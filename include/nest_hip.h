@@ -121,7 +121,7 @@ int nest_hip_sim2d_set_stream(nest_sim2d* s, void* stream);
    tensors; NULL restores the handle's own buffer for that slot. */
 int nest_hip_sim2d_bind(nest_sim2d* s, double* dev_U, double* dev_U1);
 int nest_hip_sim2d_device_ptrs(nest_sim2d* s, double** dev_U, double** dev_U1);
-/* Work decomposition: rows per work item (>= 1) and waves per block (1..16);
+/* Work decomposition: rows per work item (>= 1) and waves per block (1..4);
    0 keeps the current value. persistent != 0 caps the grid at the number of
    resident waves and loops over the work list. */
 int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int persistent);
@@ -140,14 +140,17 @@ int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host);
    stage 1: U1 = S(U); stage 2: U = 0.5*(U + S(U1)). Halos of the input must
    be filled. patch < 0: all patches in one launch; else that patch only. */
 int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch);
-/* One full RK2 step with a host dt: fill(U), stage 1, fill(U1), stage 2,
-   fill(U). */
+/* One full RK2 step with a host dt: stage 1 and stage 2 each gather their
+   halo straight from the neighbour patches / boundary through the mesh's
+   neighbour table (the same values an exchange would deliver), then one
+   halo exchange leaves U halo-complete. */
 int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt);
 /* Device-resident time stepping. start(): t = 0, dt policy per problem
    (fixed_dt > 0 holds dt fixed, else CFL with cfl), stop at t_end (use
    INFINITY for a fixed step count), and reduces the initial wave speeds.
-   advance(): nsteps steps of [dt finalize, fill, stage 1, fill, stage 2]
-   enqueued without any host synchronisation; use_graph != 0 replays one
+   advance(): nsteps steps of [dt finalize, stage 1, stage 2] (halos
+   gathered from neighbours, as in rk2_step) enqueued without any host
+   synchronisation, then one halo exchange of U; use_graph != 0 replays one
    captured step graph. status(): per-problem t, dt, steps and done flags
    (arrays of nprob; any may be NULL), blocking. */
 int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_dt);
@@ -159,9 +162,9 @@ int nest_hip_sim2d_status(nest_sim2d* s, double* t, double* dt, int* steps, int*
 int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl);
 int nest_hip_sim2d_synchronize(nest_sim2d* s);
 /* advance() without a graph, with HIP events recorded on the handle's stream
-   around each launch of the step; blocks at the end. ms[5] receives the
-   summed durations of [dt finalize, fill U, stage 1, fill U1, stage 2] over
-   the nsteps steps, *total_ms the elapsed time of the whole sequence. */
+   around each launch of the step; blocks at the end. ms[3] receives the
+   summed durations of [dt finalize, stage 1, stage 2] over the nsteps steps,
+   *total_ms the elapsed time of the whole sequence. */
 int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* total_ms);
 
 #ifdef __cplusplus

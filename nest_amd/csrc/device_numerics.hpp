@@ -111,8 +111,31 @@ __device__ __forceinline__ prim4 prim1d(double rho, double m, double E, const eo
     return w;
 }
 
-// sound speed — euler1d.hpp:54-56
+#if NEST_FAST
+// 1/sqrt(x): hardware estimate + two Newton-Raphson steps
+__device__ __forceinline__ double rsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * __builtin_fma(-h, y * y, 1.5);
+    y = y * __builtin_fma(-h, y * y, 1.5);
+    return y;
+}
+#endif
+
+// sound speed — euler1d.hpp:54-56: sqrt(gamma*p/rho). Fast mode uses
+// sqrt(gamma*p/rho) = (gamma*p) * rsq(gamma*p*rho): one reciprocal square root
+// instead of a division and a square root.
 __device__ __forceinline__ double sound_speed(double rho, double p, const eos_c& e) {
+#if NEST_FAST
+    const double gp = e.gamma * p;
+    return gp * rsq(gp * rho);
+#else
+    return dsqrt(ddiv(e.gamma * p, rho));
+#endif
+}
+// 1D variant: the 1D pressure clamp is p >= 0 (euler1d.hpp:43), so p may be
+// exactly 0 where the reciprocal square root form would give 0*inf.
+__device__ __forceinline__ double sound_speed_1d(double rho, double p, const eos_c& e) {
     return dsqrt(ddiv(e.gamma * p, rho));
 }
 // total energy — euler1d.hpp:47-51
@@ -133,9 +156,9 @@ __device__ __forceinline__ double total_energy(double rho, double u, double p, c
 // F[0..2] = HLLE (mass, normal momentum, energy); F[3] = passive flux.
 template <bool PASSIVE>
 __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const eos_c& e, double F[4]) {
-    double cL = sound_speed(L.r, L.p, e);
+    double cL = PASSIVE ? sound_speed(L.r, L.p, e) : sound_speed_1d(L.r, L.p, e);
     double EL = total_energy(L.r, L.u, L.p, e);
-    double cR = sound_speed(R.r, R.p, e);
+    double cR = PASSIVE ? sound_speed(R.r, R.p, e) : sound_speed_1d(R.r, R.p, e);
     double ER = total_energy(R.r, R.u, R.p, e);
     double SL = dmin(L.u - cL, R.u - cR);
     double SR = dmax(L.u + cL, R.u + cR);

@@ -69,20 +69,60 @@ __global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
         const double dtdx = dt / ps->dx;
         const double dtdy = dt / ps->dy;
 
-        const int nye = pd.ny + 4;
-        const int vs = (pd.nx + 4) * nye;
-        const int ec = w.c0 + lane;
-        const int ecl = ec < nye ? ec : nye - 1;
-        const double* in = (STAGE == 1 ? a.U : a.U1) + pd.off + ecl;
-        double* out = (STAGE == 1 ? a.U1 : a.U) + pd.off + ec;
-        const bool store = lane >= 2 && lane < 62 && ec >= 2 && ec < pd.ny + 2;
+        const double* Ubuf = STAGE == 1 ? a.U : a.U1;
+        double* Obuf = STAGE == 1 ? a.U1 : a.U;
+        // Per lane: where this lane's column lives for interior rows (base,
+        // row length, variable stride) and where its output goes.
+        const double* base;
+        double* out;
+        int snye, svs;
+        bool store;
+        int g = 0;  // global column (gather mode)
+        nest_prob_geom G{};
+        if (a.gather) {
+            // global strip: lane -> global column g -> owning patch (q) and its
+            // local column; outputs for lanes 2..61 inside the domain
+            G = a.geoms[pd.prob];
+            g = pd.y0 + w.c0 + lane - 2;
+            int q, le, lc;
+            nest_gather_source(pd, G, a.itab, 2, g, q, le, lc);
+            const nest_patch_desc qd = a.patches[q];
+            snye = qd.ny + 4;
+            svs = (qd.nx + 4) * snye;
+            base = Ubuf + qd.off + lc;
+            out = Obuf + qd.off + lc;
+            store = lane >= 2 && lane < 62 && g < G.NY;
+        } else {
+            // per-patch strip over the patch's own extended columns (ghosts filled)
+            const int nye = pd.ny + 4;
+            const int ec = w.c0 + lane;
+            const int ecl = ec < nye ? ec : nye - 1;  // lanes past the patch: any in-bounds column
+            snye = nye;
+            svs = (pd.nx + 4) * nye;
+            base = Ubuf + pd.off + ecl;
+            out = Obuf + pd.off + ecl;
+            store = lane >= 2 && lane < 62 && ec >= 2 && ec < pd.ny + 2;
+        }
+        const int vs = svs;  // variable stride of the output patch
 
         auto load_row = [&](int e, double* u) {
-            const double* p = in + (size_t)e * nye;
+            const double* p;
+            int s;
+            if (a.gather && (e < 2 || e >= pd.nx + 2)) {  // ghost rows: wave-uniform slow path
+                int q, le, lc;
+                nest_gather_source(pd, G, a.itab, e, g, q, le, lc);
+                const nest_patch_desc qd = a.patches[q];
+                const int qnye = qd.ny + 4;
+                s = (qd.nx + 4) * qnye;
+                p = Ubuf + qd.off + (size_t)le * qnye + lc;
+            } else {
+                p = base + (size_t)e * snye;
+                s = svs;
+            }
             u[0] = ld(p);
-            u[1] = ld(p + vs);
-            u[2] = ld(p + 2 * (size_t)vs);
-            u[3] = ld(p + 3 * (size_t)vs);
+            u[1] = ld(p + s);
+            u[2] = ld(p + 2 * (size_t)s);
+            u[3] = ld(p + 3 * (size_t)s);
         };
 
         // ---- X window warm-up: rows i = r0-2 .. r0+1 (extended rows r0 .. r0+3)
@@ -104,9 +144,13 @@ __global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
         face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
 
         unsigned long long smx = 0, smy = 0;
+        double Un[4];
+        load_row(w.r0 + 4, Un);
+#pragma unroll 2
         for (int i = w.r0; i < w.r1; ++i) {
-            double Un[4];
-            load_row(i + 4, Un);
+            // prefetch the row the next iteration needs (clamped in bounds on the last)
+            double Unx[4];
+            load_row(min(i + 5, pd.nx + 3), Unx);
             prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
             prim4 dW2 = sub4(W2, W1);
             prim4 s1 = slope4<LIM>(dW1, dW2);
@@ -126,6 +170,7 @@ __global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
                 Fm[k] = Fp[k];
                 Ua[k] = Ub[k];
                 Ub[k] = Un[k];
+                Un[k] = Unx[k];
             }
 
             // ---- Y-sweep of the X-swept row across lanes (euler2d.hpp:176-236)
@@ -143,7 +188,7 @@ __global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
             double o1 = ux1 - dtdy * (G[3] - Gm[3]);
             double o2 = ux2 - dtdy * (G[1] - Gm[1]);
             double o3 = ux3 - dtdy * (G[2] - Gm[2]);
-            double* po = out + (size_t)(i + 2) * nye;
+            double* po = out + (size_t)(i + 2) * snye;
             if constexpr (STAGE == 2) {
                 // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2), in place of U^n
                 if (store) {
@@ -327,7 +372,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) o[k] = 0.5 * (out[k * ne + e] + o[k]);
             prim4 w = prim1d(o[0], o[1], o[2], a.eos);
-            double s = __builtin_fabs(w.u) + sound_speed(w.r, w.p, a.eos);
+            double s = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
             sm = (unsigned long long)__double_as_longlong(s);
         }
 #pragma unroll
@@ -345,7 +390,7 @@ __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
     unsigned long long sm = 0;
     if (e < n + 2) {
         prim4 w = prim1d(a.U[e], a.U[ne + e], a.U[2 * ne + e], a.eos);
-        double s = __builtin_fabs(w.u) + sound_speed(w.r, w.p, a.eos);
+        double s = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
         sm = (unsigned long long)__double_as_longlong(s);
     }
     sm = wave_max_u64(sm);

@@ -29,6 +29,12 @@ struct stage_args {
     nest_prob_state* probs;
     nest_eos eos;
     double host_dt;  // > 0: use this dt for every problem (ignores done flags)
+    // gather != 0: items are global strips and every cell (halo included) is
+    // read from the patch that owns it (nest_gather_source); 0: items are
+    // per-patch strips reading the patch's own ghost cells.
+    int gather;
+    const nest_prob_geom* geoms;
+    const int* itab;
 };
 
 // arguments of the 1D kernels (one problem, one patch)

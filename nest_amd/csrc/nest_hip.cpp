@@ -106,10 +106,12 @@ struct nest_sim2d {
     nest_eos eos{};
     int nprob = 0;
     std::vector<nest_prob_state> probs_h;
-    std::vector<nest_patch_desc> patches_h;
-    std::vector<nest_region> regions_h;
-    std::vector<nest_item> items_h;
-    std::vector<int> item_begin, item_end;  // per patch
+    nest_plan::tables tb;  // patches, halo regions, geometry, owner tables
+    std::vector<nest_patch_desc>& patches_h = tb.patches;
+    std::vector<nest_region>& regions_h = tb.regions;
+    std::vector<nest_item> items_h;         // per-patch strips (own ghosts)
+    std::vector<int> item_begin, item_end;  // per patch, into items_h
+    std::vector<nest_item> gitems_h;        // global strips (halo gathered from neighbours)
     size_t total = 0;
 
     double* U_own = nullptr;
@@ -120,6 +122,9 @@ struct nest_sim2d {
     nest_region* regions_d = nullptr;
     nest_item* items_d = nullptr;
     nest_prob_state* probs_d = nullptr;
+    nest_item* gitems_d = nullptr;
+    nest_prob_geom* geoms_d = nullptr;
+    int* itab_d = nullptr;
 
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -142,10 +147,13 @@ struct nest_sim2d {
     void release() {
         drop_graph();
         for (void* p : {(void*)U_own, (void*)U1_own, (void*)patches_d, (void*)regions_d, (void*)items_d,
-                        (void*)probs_d})
+                        (void*)probs_d, (void*)gitems_d, (void*)geoms_d, (void*)itab_d})
             if (p) (void)hipFree(p);
         U_own = U1_own = U = U1 = nullptr;
         patches_d = nullptr;
+        gitems_d = nullptr;
+        geoms_d = nullptr;
+        itab_d = nullptr;
         regions_d = nullptr;
         items_d = nullptr;
         probs_d = nullptr;
@@ -155,16 +163,22 @@ struct nest_sim2d {
 
     const launch_table& L() const { return table(mode); }
 
-    stage_args args(double host_dt, int item0, int nitems) const {
+    // gather != 0: global strips (gitems) reading every cell, halo included,
+    // from the patch that owns it (no exchange needed before the stage);
+    // 0: per-patch strips (items) reading the patch's own ghost cells.
+    stage_args args(double host_dt, int item0, int nitems, int gather = 0) const {
         stage_args a{};
         a.U = U;
         a.U1 = U1;
         a.patches = patches_d;
-        a.items = items_d + item0;
+        a.items = (gather ? gitems_d : items_d) + item0;
         a.nitems = nitems;
         a.probs = probs_d;
         a.eos = eos;
         a.host_dt = host_dt;
+        a.gather = gather;
+        a.geoms = geoms_d;
+        a.itab = itab_d;
         return a;
     }
     int grid_for(int nitems) const {
@@ -180,14 +194,19 @@ struct nest_sim2d {
         item_end.assign(patches_h.size(), 0);
         for (size_t p = 0; p < patches_h.size(); ++p) {
             item_begin[p] = (int)items_h.size();
-            nest_plan::add_items(items_h, (int)p, patches_h[p].nx, patches_h[p].ny, rows);
+            nest_plan::add_patch_items(items_h, (int)p, patches_h[p].nx, patches_h[p].ny, rows);
             item_end[p] = (int)items_h.size();
         }
+        gitems_h.clear();
+        for (int q = 0; q < (int)tb.geoms.size(); ++q) nest_plan::add_gather_items(gitems_h, tb, q, rows);
         if (int rc = validate_tables()) return rc;
-        if (items_d) (void)hipFree(items_d);
-        items_d = nullptr;
+        for (nest_item** d : {&items_d, &gitems_d})
+            if (*d) (void)hipFree(*d);
+        items_d = gitems_d = nullptr;
         if (int rc = dev_alloc(&items_d, items_h.size())) return rc;
+        if (int rc = dev_alloc(&gitems_d, gitems_h.size())) return rc;
         NEST_HIP_TRY(hipMemcpy(items_d, items_h.data(), items_h.size() * sizeof(nest_item), hipMemcpyHostToDevice));
+        NEST_HIP_TRY(hipMemcpy(gitems_d, gitems_h.data(), gitems_h.size() * sizeof(nest_item), hipMemcpyHostToDevice));
         drop_graph();
         return NEST_OK;
     }
@@ -195,7 +214,8 @@ struct nest_sim2d {
     // Every device table entry must address memory inside the buffers: checked
     // on the host before anything is uploaded or launched.
     int validate_tables() const {
-        if (!nest_plan::tables_in_bounds(patches_h, regions_h, items_h, (long long)total))
+        if (!nest_plan::tables_in_bounds(tb, items_h) || !nest_plan::tables_in_bounds(tb, gitems_h) ||
+            (long long)total != tb.total)
             return fail(NEST_ERR_INVALID, "internal: a patch / halo region / work item addresses memory "
                                           "outside the buffers");
         return NEST_OK;
@@ -212,6 +232,11 @@ struct nest_sim2d {
         if (int rc = dev_alloc(&patches_d, patches_h.size())) return rc;
         if (int rc = dev_alloc(&regions_d, regions_h.size())) return rc;
         if (int rc = dev_alloc(&probs_d, probs_h.size())) return rc;
+        if (int rc = dev_alloc(&geoms_d, tb.geoms.size())) return rc;
+        if (int rc = dev_alloc(&itab_d, tb.itab.size())) return rc;
+        NEST_HIP_TRY(hipMemcpy(geoms_d, tb.geoms.data(), tb.geoms.size() * sizeof(nest_prob_geom),
+                               hipMemcpyHostToDevice));
+        NEST_HIP_TRY(hipMemcpy(itab_d, tb.itab.data(), tb.itab.size() * sizeof(int), hipMemcpyHostToDevice));
         NEST_HIP_TRY(hipMemcpy(patches_d, patches_h.data(), patches_h.size() * sizeof(nest_patch_desc),
                                hipMemcpyHostToDevice));
         if (!regions_h.empty())
@@ -230,20 +255,21 @@ struct nest_sim2d {
 
     // ---- one launch of each step component ----
     void fill(double* base) { L().fill_regions(base, regions_d, (int)regions_h.size(), 4, stream); }
-    void stage(int st, double host_dt, int item0, int nitems) {
+    void stage(int st, double host_dt, int item0, int nitems, int gather = 0) {
         if (nitems <= 0) return;
-        L().stage2d(st, lim, args(host_dt, item0, nitems), grid_for(nitems), 64 * wpb, stream);
+        L().stage2d(st, lim, args(host_dt, item0, nitems, gather), grid_for(nitems), 64 * wpb, stream);
     }
     void speeds(int item0, int nitems) {
         if (nitems <= 0) return;
         L().speeds2d(args(0.0, item0, nitems), grid_for(nitems), 64 * wpb, stream);
     }
+    // One fused step: device dt, then the two stages gathering their halos
+    // from the neighbours (no exchange launches). Ghost cells of U are left
+    // stale; callers refresh them with fill(U) when they hand U out.
     void device_step() {
         L().finalize(probs_d, nprob, stream);
-        fill(U);
-        stage(1, 0.0, 0, (int)items_h.size());
-        fill(U1);
-        stage(2, 0.0, 0, (int)items_h.size());
+        stage(1, 0.0, 0, (int)gitems_h.size(), 1);
+        stage(2, 0.0, 0, (int)gitems_h.size(), 1);
     }
 };
 
@@ -294,9 +320,8 @@ int nest_hip_mesh2d_create(int nx, int ny, int npx, int npy, double dx, double d
     s->eos = make_eos(gamma);
     s->nprob = 1;
     s->probs_h.push_back(make_prob(dx, dy, 2));
-    long long total = 0;
-    build_mesh(nx, ny, npx, npy, bcx, bcy, 0, s->patches_h, s->regions_h, total);
-    s->total = (size_t)total;
+    build_mesh(s->tb, nx, ny, npx, npy, bcx, bcy);
+    s->total = (size_t)s->tb.total;
     if (int rc = s->init_device()) {
         delete s;
         return rc;
@@ -317,16 +342,15 @@ int nest_hip_batch2d_create(int nprob, const int* nx, const int* ny, const doubl
     s->lim = limiter;
     s->eos = make_eos(gamma);
     s->nprob = nprob;
-    long long total = 0;
     for (int i = 0; i < nprob; ++i) {
         if (nx[i] < 2 || ny[i] < 2 || !(dx[i] > 0.0) || !(dy[i] > 0.0)) {
             delete s;
             return fail(NEST_ERR_INVALID, "problem " + std::to_string(i) + ": bad size or spacing");
         }
         s->probs_h.push_back(make_prob(dx[i], dy[i], 2));
-        build_mesh(nx[i], ny[i], 1, 1, bc, bc, i, s->patches_h, s->regions_h, total);
+        build_mesh(s->tb, nx[i], ny[i], 1, 1, bc, bc);
     }
-    s->total = (size_t)total;
+    s->total = (size_t)s->tb.total;
     if (int rc = s->init_device()) {
         delete s;
         return rc;
@@ -386,8 +410,9 @@ int nest_hip_sim2d_device_ptrs(nest_sim2d* s, double** dev_U, double** dev_U1) {
 
 int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int persistent) {
     if (int rc = valid_sim(s)) return rc;
-    if (rows < 0 || waves_per_block < 0 || waves_per_block > 16)
-        return fail(NEST_ERR_INVALID, "rows >= 1, 1 <= waves_per_block <= 16");
+    // the stage kernels are compiled with __launch_bounds__(256): at most 4 waves per block
+    if (rows < 0 || waves_per_block < 0 || waves_per_block > 4)
+        return fail(NEST_ERR_INVALID, "rows >= 1, 1 <= waves_per_block <= 4");
     if (waves_per_block) s->wpb = waves_per_block;
     s->persistent = persistent;
     if (rows && rows != s->rows) {
@@ -467,11 +492,9 @@ int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch) {
 int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt) {
     if (int rc = valid_sim(s)) return rc;
     if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
-    const int n = (int)s->items_h.size();
-    s->fill(s->U);
-    s->stage(1, dt, 0, n);
-    s->fill(s->U1);
-    s->stage(2, dt, 0, n);
+    const int n = (int)s->gitems_h.size();
+    s->stage(1, dt, 0, n, 1);
+    s->stage(2, dt, 0, n, 1);
     s->fill(s->U);
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
@@ -503,17 +526,18 @@ int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph) {
     if (nsteps < 0) return fail(NEST_ERR_INVALID, "nsteps < 0");
     if (!use_graph) {
         for (int k = 0; k < nsteps; ++k) s->device_step();
-        NEST_HIP_TRY(hipGetLastError());
-        return NEST_OK;
+    } else {
+        if (!s->graph_exec) {
+            NEST_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            s->device_step();
+            hipError_t e = hipStreamEndCapture(s->stream, &s->graph);
+            if (e != hipSuccess) return fail(NEST_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            NEST_HIP_TRY(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+        }
+        for (int k = 0; k < nsteps; ++k) NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
     }
-    if (!s->graph_exec) {
-        NEST_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        s->device_step();
-        hipError_t e = hipStreamEndCapture(s->stream, &s->graph);
-        if (e != hipSuccess) return fail(NEST_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        NEST_HIP_TRY(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
-    }
-    for (int k = 0; k < nsteps; ++k) NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+    if (nsteps > 0) s->fill(s->U);  // the fused steps leave U's ghost cells stale
+    NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
 
@@ -563,36 +587,32 @@ int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl) {
 
 int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* total_ms) {
     if (int rc = valid_sim(s)) return rc;
-    if (nsteps < 1 || !ms) return fail(NEST_ERR_INVALID, "nsteps >= 1 and ms[5] required");
-    std::vector<hipEvent_t> ev(6 * (size_t)nsteps, nullptr);
+    if (nsteps < 1 || !ms) return fail(NEST_ERR_INVALID, "nsteps >= 1 and ms[3] required");
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
     int rc = NEST_OK;
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) rc = fail(NEST_ERR_HIP, "hipEventCreate failed");
     if (!rc) {
-        const int n = (int)s->items_h.size();
-        for (int k = 0; k < nsteps; ++k) {
-            hipEvent_t* e = &ev[6 * (size_t)k];
+        const int n = (int)s->gitems_h.size();
+        for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(), with events between
+            hipEvent_t* e = &ev[4 * (size_t)k];
             (void)hipEventRecord(e[0], s->stream);
             s->L().finalize(s->probs_d, s->nprob, s->stream);
             (void)hipEventRecord(e[1], s->stream);
-            s->fill(s->U);
+            s->stage(1, 0.0, 0, n, 1);
             (void)hipEventRecord(e[2], s->stream);
-            s->stage(1, 0.0, 0, n);
+            s->stage(2, 0.0, 0, n, 1);
             (void)hipEventRecord(e[3], s->stream);
-            s->fill(s->U1);
-            (void)hipEventRecord(e[4], s->stream);
-            s->stage(2, 0.0, 0, n);
-            (void)hipEventRecord(e[5], s->stream);
         }
         hipError_t e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) rc = fail(NEST_ERR_HIP, std::string("advance_timed: ") + hipGetErrorString(e));
     }
     if (!rc) {
-        for (int c = 0; c < 5; ++c) ms[c] = 0.0;
+        for (int c = 0; c < 3; ++c) ms[c] = 0.0;
         for (int k = 0; k < nsteps; ++k)
-            for (int c = 0; c < 5; ++c) {
+            for (int c = 0; c < 3; ++c) {
                 float t = 0.f;
-                (void)hipEventElapsedTime(&t, ev[6 * (size_t)k + c], ev[6 * (size_t)k + c + 1]);
+                (void)hipEventElapsedTime(&t, ev[4 * (size_t)k + c], ev[4 * (size_t)k + c + 1]);
                 ms[c] += t;
             }
         if (total_ms) {
@@ -603,6 +623,7 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
     }
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    if (!rc) s->fill(s->U);  // as advance(): hand out a halo-complete U
     return rc;
 }
 

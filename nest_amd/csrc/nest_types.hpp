@@ -1,18 +1,36 @@
-// Plain structs shared by the host runtime (nest_hip.cpp) and the device
-// kernels (kernels.hip). Everything here is trivially copyable and lives in
-// device memory as flat tables.
+// Plain structs shared by the host runtime (nest_hip.cpp), the host-side plan
+// builder (plan.hpp) and the device kernels (kernels.hip). Everything here is
+// trivially copyable and lives in device memory as flat tables.
 #pragma once
 #include <cstdint>
 
+#if defined(__HIPCC__)
+#define NEST_HD __host__ __device__
+#else
+#define NEST_HD
+#endif
+
 // One rectangular patch of a problem: its field_t array (extended space,
 // 2-cell halo, variable-major, last index contiguous) starts at element
-// offset `off` of the problem buffers U / U1.
+// offset `off` of the problem-set buffers U / U1.
 struct nest_patch_desc {
     long long off;  // element offset of the patch array in U and U1
     int nx;         // interior cells along dim 0 (x, strided)
     int ny;         // interior cells along dim 1 (y, contiguous); 1D: unused
     int prob;       // index of the independent problem the patch belongs to
-    int pad;
+    int px;         // partition index of the patch along dim 0 within its problem
+    int x0, y0;     // global index of the first interior cell (problem coordinates)
+};
+
+// Static geometry of one problem: an npx x npy patch grid over NX x NY cells.
+// The problem-set int table holds, from `tab`: owner_x[NX] (partition index
+// of every global row), owner_y[NY], then the partition starts x0[npx],
+// y0[npy]. Patch (qx, qy) is patch_base + qy*npx + qx.
+struct nest_prob_geom {
+    int patch_base, npx, npy;
+    int NX, NY;
+    int bcx, bcy;  // 0 periodic, 1 outflow (zero-gradient)
+    int tab;
 };
 
 // Per-problem time-stepping state, resident on the device so a whole step
@@ -51,8 +69,37 @@ struct nest_eos {
     double gamma, gm1, inv_gm1;
 };
 
-// A unit of stage work: patch `patch`, a strip of 64 extended columns starting
-// at c0 (outputs for the 60 middle ones), interior rows [r0, r1).
+// A unit of stage work: a strip of 64 consecutive extended columns starting
+// at extended column c0 of patch `patch` (outputs for the 60 middle lanes),
+// interior rows [r0, r1) of that patch. In gather mode a strip may run past
+// the patch's last column into its dim-1 neighbours (global strips).
 struct nest_item {
     int patch, c0, r0, r1;
 };
+
+// Global index mapped through a boundary condition: periodic wrap or
+// zero-gradient (outflow) clamp.
+NEST_HD inline int nest_map_index(int g, int n, int bc) {
+    if (bc == 0) {
+        int r = g % n;
+        return r < 0 ? r + n : r;
+    }
+    return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+// Where the value of extended row e of patch p, at global column g of its
+// problem, lives: patch q, extended row le, extended column lc of q (always
+// an interior cell of q). Ghost rows/columns resolve to the neighbour patch
+// holding that global cell, or to the boundary cell (outflow) / wrapped cell
+// (periodic) — faces and corners alike. Host plan checks and the device
+// gather use this same arithmetic.
+NEST_HD inline void nest_gather_source(const nest_patch_desc& pd, const nest_prob_geom& G, const int* T, int e,
+                                       int g, int& q, int& le, int& lc) {
+    const int gc = nest_map_index(g, G.NY, G.bcy);
+    const int qy = T[G.tab + G.NX + gc];
+    lc = gc - T[G.tab + G.NX + G.NY + G.npx + qy] + 2;
+    const int gr = nest_map_index(pd.x0 + e - 2, G.NX, G.bcx);
+    const int qx = T[G.tab + gr];
+    le = gr - T[G.tab + G.NX + G.NY + qx] + 2;
+    q = G.patch_base + qy * G.npx + qx;
+}

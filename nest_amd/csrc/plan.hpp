@@ -1,8 +1,9 @@
 // Host-side construction of the device tables of a 2D problem set: the
-// tensor-product patch partition, the halo exchange plan (faces + corners,
-// periodic / outflow / patch-to-patch) and the stage work list. Pure C++ (no
-// HIP), so the exact tables the GPU consumes are also checked on the CPU
-// (tests/cpp/test_plan.cpp executes them against the oracle's halo fill).
+// tensor-product patch partition, the per-problem geometry and owner tables,
+// the halo exchange plan (faces + corners, periodic / outflow /
+// patch-to-patch) and the stage work lists. Pure C++ (no HIP), so the exact
+// tables the GPU consumes are also checked on the CPU (tests/cpp/
+// test_plan.cpp runs them against the oracle's halo fill).
 #pragma once
 #include <algorithm>
 #include <vector>
@@ -13,13 +14,13 @@ namespace nest_plan {
 
 enum { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
 
-// halo regions of one patch (px, py) of a tensor-product mesh: faces and
-// corners, each mapped through the per-dimension boundary condition.
+// Tensor-product partition of one dimension with subspace() semantics
+// (reference include/nest/core.hpp:184-199).
 struct dim_part {
     std::vector<int> start, len;
 };
 
-inline dim_part partition(int total, int n) {  // subspace() semantics, reference core.hpp:184-199
+inline dim_part partition(int total, int n) {
     dim_part d;
     const int base = total / n, extra = total % n;
     for (int p = 0; p < n; ++p) {
@@ -59,10 +60,10 @@ inline void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& 
     }
 }
 
-// off[] holds the array offsets of this mesh's patches in local patch-index
-// order (py*npx + px).
+// Halo regions (faces + corners) of every patch of one mesh; off[] holds the
+// array offsets of the mesh's patches in local patch-index order (py*npx+px).
 inline void add_mesh_regions(std::vector<nest_region>& R, const dim_part& X, const dim_part& Y, int bcx, int bcy,
-                      const std::vector<long long>& off) {
+                             const std::vector<long long>& off) {
     const int npx = (int)X.len.size(), npy = (int)Y.len.size();
     const int h = 2;
     for (int py = 0; py < npy; ++py)
@@ -92,36 +93,85 @@ inline void add_mesh_regions(std::vector<nest_region>& R, const dim_part& X, con
         }
 }
 
-// Append one problem's patch mesh (nx*ny cells on npx*npy patches) to the
-// problem-set tables: patch descriptors at consecutive buffer offsets from
-// `total` (advanced), and its halo regions.
-inline void build_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int prob,
-                       std::vector<nest_patch_desc>& P, std::vector<nest_region>& R, long long& total) {
+// All device tables of a problem set.
+struct tables {
+    std::vector<nest_patch_desc> patches;
+    std::vector<nest_region> regions;
+    std::vector<nest_prob_geom> geoms;
+    std::vector<int> itab;  // owner / start tables of every problem (see nest_prob_geom)
+    long long total = 0;    // doubles per buffer (U or U1)
+};
+
+// Append one problem's patch mesh (nx*ny cells on npx*npy patches): patch
+// descriptors at consecutive buffer offsets, halo regions, geometry, owner
+// tables.
+inline void build_mesh(tables& t, int nx, int ny, int npx, int npy, int bcx, int bcy) {
+    const int prob = (int)t.geoms.size();
     dim_part X = partition(nx, npx), Y = partition(ny, npy);
+    nest_prob_geom G{};
+    G.patch_base = (int)t.patches.size();
+    G.npx = npx;
+    G.npy = npy;
+    G.NX = nx;
+    G.NY = ny;
+    G.bcx = bcx;
+    G.bcy = bcy;
+    G.tab = (int)t.itab.size();
+    for (int px = 0; px < npx; ++px)
+        for (int i = 0; i < X.len[px]; ++i) t.itab.push_back(px);
+    for (int py = 0; py < npy; ++py)
+        for (int j = 0; j < Y.len[py]; ++j) t.itab.push_back(py);
+    for (int px = 0; px < npx; ++px) t.itab.push_back(X.start[px]);
+    for (int py = 0; py < npy; ++py) t.itab.push_back(Y.start[py]);
+    t.geoms.push_back(G);
     std::vector<long long> off;
     for (int py = 0; py < npy; ++py)
         for (int px = 0; px < npx; ++px) {
-            off.push_back(total);
-            P.push_back({total, X.len[px], Y.len[py], prob, 0});
-            total += 4LL * (X.len[px] + 4) * (Y.len[py] + 4);
+            off.push_back(t.total);
+            t.patches.push_back({t.total, X.len[px], Y.len[py], prob, px, X.start[px], Y.start[py]});
+            t.total += 4LL * (X.len[px] + 4) * (Y.len[py] + 4);
         }
-    add_mesh_regions(R, X, Y, bcx, bcy, off);
+    add_mesh_regions(t.regions, X, Y, bcx, bcy, off);
 }
 
-// Work list of one patch: row chunks of `rows`, strips of 60 output columns
-// (64-lane strips over the extended columns, 2 halo lanes each side).
-inline void add_items(std::vector<nest_item>& items, int patch, int nx, int ny, int rows) {
+// Per-patch work list (halo read from the patch's own ghost cells): row
+// chunks of `rows`, strips of 60 output columns over the patch.
+inline void add_patch_items(std::vector<nest_item>& items, int patch, int nx, int ny, int rows) {
     for (int r0 = 0; r0 < nx; r0 += rows)
         for (int c0 = 0; c0 < ny; c0 += 60) items.push_back({patch, c0, r0, std::min(r0 + rows, nx)});
 }
 
+// Global-strip work list of one problem (halo gathered from the neighbours):
+// for every dim-0 partition and row chunk, strips of 60 output columns over
+// the problem's whole dim-1 range, each anchored in the patch holding its
+// first output column. A strip's lanes may run into the next patches.
+inline void add_gather_items(std::vector<nest_item>& items, const tables& t, int prob, int rows) {
+    const nest_prob_geom& G = t.geoms[prob];
+    const int* T = t.itab.data() + G.tab;
+    for (int px = 0; px < G.npx; ++px) {
+        const int lx = t.patches[G.patch_base + px].nx;
+        for (int r0 = 0; r0 < lx; r0 += rows)
+            for (int gs = 0; gs < G.NY; gs += 60) {
+                const int qy = T[G.NX + gs];
+                const int home = G.patch_base + qy * G.npx + px;
+                const int c0 = gs - T[G.NX + G.NY + G.npx + qy];
+                items.push_back({home, c0, r0, std::min(r0 + rows, lx)});
+            }
+    }
+}
+
 // Every table entry addresses memory inside a buffer of `total` doubles.
-inline bool tables_in_bounds(const std::vector<nest_patch_desc>& P, const std::vector<nest_region>& R,
-                             const std::vector<nest_item>& I, long long total) {
-    auto arr_ok = [&](long long off, long long vs) { return off >= 0 && off + 4 * vs <= total; };
-    for (const auto& d : P)
-        if (d.nx < 2 || d.ny < 2 || !arr_ok(d.off, (long long)(d.nx + 4) * (d.ny + 4))) return false;
-    for (const auto& g : R) {
+inline bool tables_in_bounds(const tables& t, const std::vector<nest_item>& I) {
+    auto arr_ok = [&](long long off, long long vs) { return off >= 0 && off + 4 * vs <= t.total; };
+    for (const auto& d : t.patches)
+        if (d.nx < 2 || d.ny < 2 || d.prob < 0 || d.prob >= (int)t.geoms.size() ||
+            !arr_ok(d.off, (long long)(d.nx + 4) * (d.ny + 4)))
+            return false;
+    for (const auto& G : t.geoms)
+        if (G.tab < 0 || G.tab + G.NX + G.NY + G.npx + G.npy > (int)t.itab.size() || G.patch_base < 0 ||
+            G.patch_base + G.npx * G.npy > (int)t.patches.size())
+            return false;
+    for (const auto& g : t.regions) {
         const int dnx = g.dst_vs / g.dst_nye, snx = g.src_vs / g.src_nye;
         bool ok = arr_ok(g.dst_off, g.dst_vs) && arr_ok(g.src_off, g.src_vs) && g.ni > 0 && g.nj > 0 &&
                   g.di >= 0 && g.di + g.ni <= dnx && g.dj >= 0 && g.dj + g.nj <= g.dst_nye && g.si >= 0 &&
@@ -129,8 +179,8 @@ inline bool tables_in_bounds(const std::vector<nest_patch_desc>& P, const std::v
         if (!ok) return false;
     }
     for (const auto& it : I) {
-        if (it.patch < 0 || it.patch >= (int)P.size()) return false;
-        const auto& d = P[it.patch];
+        if (it.patch < 0 || it.patch >= (int)t.patches.size()) return false;
+        const auto& d = t.patches[it.patch];
         if (it.r0 < 0 || it.r0 >= it.r1 || it.r1 > d.nx || it.c0 < 0 || it.c0 >= d.ny) return false;
     }
     return true;

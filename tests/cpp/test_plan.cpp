@@ -1,8 +1,15 @@
 // CPU check of the device tables (nest_amd/csrc/plan.hpp): for meshes and
-// batches of every shape the library builds, the halo-region table and the
-// stage work list stay inside the buffers, and executing the region table
-// reproduces the oracle's halo fill (faces + corners, periodic / outflow)
-// bit for bit. Exit status 0 = pass.
+// batches of every shape the library builds,
+//  * the halo-region table, the geometry / owner tables and both work lists
+//    stay inside the buffers;
+//  * executing the region table reproduces the oracle's halo fill (faces +
+//    corners, periodic / outflow) bit for bit;
+//  * nest_gather_source (the addressing of the stage kernels' in-kernel halo
+//    gather) names, for every extended cell of every patch, an interior cell
+//    that holds the oracle's ghost value;
+//  * the per-patch strips and the global strips each write every interior
+//    cell exactly once.
+// Exit status 0 = pass.
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -11,49 +18,101 @@
 #include "../../oracle/euler_oracle.hpp"
 
 static int failures = 0;
-#define CHECK(c, ...)                        \
-    do {                                     \
-        if (!(c)) {                          \
-            std::printf("FAIL: " __VA_ARGS__); \
-            std::printf("\n");               \
-            ++failures;                      \
-        }                                    \
+#define CHECK(c, ...)                          \
+    do {                                       \
+        if (!(c)) {                            \
+            if (failures < 20) {               \
+                std::printf("FAIL: " __VA_ARGS__); \
+                std::printf("\n");             \
+            }                                  \
+            ++failures;                        \
+        }                                      \
     } while (0)
 
-static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int rows) {
-    std::vector<nest_patch_desc> P;
-    std::vector<nest_region> R;
-    std::vector<nest_item> I;
-    long long total = 0;
-    nest_plan::build_mesh(nx, ny, npx, npy, bcx, bcy, 0, P, R, total);
-    for (int p = 0; p < (int)P.size(); ++p) nest_plan::add_items(I, p, P[p].nx, P[p].ny, rows);
-    CHECK(nest_plan::tables_in_bounds(P, R, I, total), "bounds %dx%d %dx%d bc %d%d", nx, ny, npx, npy, bcx, bcy);
-    CHECK(R.size() == 8 * P.size(), "region count");
-    // every interior cell is owned by exactly one work item
-    std::vector<int> cover(total / 4, 0);
+using nest_plan::tables;
+
+// lanes 2..61 of every item: which interior cells do they write?
+static void check_coverage(const tables& t, const std::vector<nest_item>& I, bool gather, const char* what) {
+    std::vector<int> cover(t.total / 4, 0);
     for (const auto& it : I) {
-        const auto& d = P[it.patch];
-        for (int i = it.r0; i < it.r1; ++i)
-            for (int lane = 2; lane < 62; ++lane) {
-                int ec = it.c0 + lane;
-                if (ec >= 2 && ec < d.ny + 2) cover[(d.off / 4) + (long long)(i + 2) * (d.ny + 4) + ec]++;
+        const auto& pd = t.patches[it.patch];
+        const auto& G = t.geoms[pd.prob];
+        for (int lane = 2; lane < 62; ++lane) {
+            long long off;
+            int lc, nye;
+            if (gather) {
+                const int g = pd.y0 + it.c0 + lane - 2;
+                if (g >= G.NY) continue;
+                int q, le;
+                nest_gather_source(pd, G, t.itab.data(), 2, g, q, le, lc);
+                off = t.patches[q].off;
+                nye = t.patches[q].ny + 4;
+            } else {
+                const int ec = it.c0 + lane;
+                if (ec < 2 || ec >= pd.ny + 2) continue;
+                off = pd.off;
+                lc = ec;
+                nye = pd.ny + 4;
             }
+            for (int i = it.r0; i < it.r1; ++i) cover[off / 4 + (long long)(i + 2) * nye + lc]++;
+        }
     }
-    for (const auto& d : P)
+    for (const auto& d : t.patches)
         for (int i = 0; i < d.nx; ++i)
             for (int j = 0; j < d.ny; ++j)
-                CHECK(cover[(d.off / 4) + (long long)(i + 2) * (d.ny + 4) + j + 2] == 1, "coverage");
+                CHECK(cover[d.off / 4 + (long long)(i + 2) * (d.ny + 4) + j + 2] == 1, "%s coverage", what);
+}
+
+static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int rows) {
+    tables t;
+    nest_plan::build_mesh(t, nx, ny, npx, npy, bcx, bcy);
+    std::vector<nest_item> I, GI;
+    for (int p = 0; p < (int)t.patches.size(); ++p) nest_plan::add_patch_items(I, p, t.patches[p].nx, t.patches[p].ny, rows);
+    nest_plan::add_gather_items(GI, t, 0, rows);
+    CHECK(nest_plan::tables_in_bounds(t, I) && nest_plan::tables_in_bounds(t, GI), "bounds %dx%d %dx%d bc %d%d", nx,
+          ny, npx, npy, bcx, bcy);
+    CHECK(t.regions.size() == 8 * t.patches.size(), "region count");
+    check_coverage(t, I, false, "patch strips");
+    check_coverage(t, GI, true, "global strips");
     // region execution == oracle fill
     nest_oracle::mesh2d m(nx, ny, npx, npy, 2, bcx, bcy);
-    CHECK((long long)m.total == total, "total %lld vs %zu", total, m.total);
+    CHECK((long long)m.total == t.total, "total %lld vs %zu", t.total, m.total);
     std::mt19937_64 g(nx * 7919 + ny);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
-    std::vector<double> a(total);
+    std::vector<double> a(t.total);
     for (auto& x : a) x = U(g);
     std::vector<double> b = a;
-    nest_plan::execute_regions_host(a.data(), R, 4);
+    nest_plan::execute_regions_host(a.data(), t.regions, 4);
     nest_oracle::fill_mesh(b.data(), m);
     CHECK(a == b, "fill %dx%d %dx%d bc %d%d", nx, ny, npx, npy, bcx, bcy);
+    // gather addressing == oracle ghost values, every extended cell, and the
+    // same for lanes far past the patch edge (global strips)
+    const auto& G = t.geoms[0];
+    for (int p = 0; p < (int)t.patches.size(); ++p) {
+        const auto& d = t.patches[p];
+        for (int e = 0; e < d.nx + 4; ++e)
+            for (int ec = 0; ec < d.ny + 4; ++ec) {
+                int q, le, lc;
+                nest_gather_source(d, G, t.itab.data(), e, d.y0 + ec - 2, q, le, lc);
+                const auto& qd = t.patches[q];
+                bool inside = q >= 0 && q < (int)t.patches.size() && le >= 2 && le < qd.nx + 2 && lc >= 2 &&
+                              lc < qd.ny + 2;
+                CHECK(inside, "gather source not interior p%d e%d ec%d", p, e, ec);
+                if (!inside) continue;
+                for (int k = 0; k < 4; ++k) {
+                    double got = b[qd.off + (long long)k * (qd.nx + 4) * (qd.ny + 4) + (long long)le * (qd.ny + 4) + lc];
+                    double want = b[d.off + (long long)k * (d.nx + 4) * (d.ny + 4) + (long long)e * (d.ny + 4) + ec];
+                    CHECK(got == want, "gather value p%d e%d ec%d", p, e, ec);
+                }
+            }
+        for (int e : {0, 1, 2, d.nx + 1, d.nx + 2, d.nx + 3})
+            for (int gg = d.y0 - 2; gg < d.y0 + d.ny + 66; ++gg) {
+                int q, le, lc;
+                nest_gather_source(d, G, t.itab.data(), e, gg, q, le, lc);
+                const auto& qd = t.patches[q];
+                CHECK(le >= 2 && le < qd.nx + 2 && lc >= 2 && lc < qd.ny + 2, "far gather in bounds");
+            }
+    }
 }
 
 int main() {
@@ -65,39 +124,42 @@ int main() {
             check_mesh(16, 12, 2, 2, bcx, bcy, 3);
             check_mesh(9, 7, 4, 3, bcx, bcy, 5);  // uneven, minimum-size patches
             check_mesh(256, 256, 4, 4, bcx, bcy, 64);
+            check_mesh(128, 300, 2, 5, bcx, bcy, 64);  // strips crossing several patches
         }
     // batch: independent single-patch problems appended to one buffer
     {
-        std::vector<nest_patch_desc> P;
-        std::vector<nest_region> R;
-        std::vector<nest_item> I;
-        long long total = 0;
+        tables t;
         int sizes[] = {64, 81, 100, 130, 77, 512, 2, 3};
-        for (int k = 0; k < 8; ++k) nest_plan::build_mesh(sizes[k], sizes[k] + 1, 1, 1, 1, 1, k, P, R, total);
-        for (int p = 0; p < (int)P.size(); ++p) nest_plan::add_items(I, p, P[p].nx, P[p].ny, 64);
-        CHECK(nest_plan::tables_in_bounds(P, R, I, total), "batch bounds");
-        for (int k = 0; k < 8; ++k) CHECK(P[k].prob == k, "batch prob index");
-        std::vector<double> a(total), b;
+        for (int k = 0; k < 8; ++k) nest_plan::build_mesh(t, sizes[k], sizes[k] + 1, 1, 1, 1, 1);
+        std::vector<nest_item> I, GI;
+        for (int p = 0; p < (int)t.patches.size(); ++p)
+            nest_plan::add_patch_items(I, p, t.patches[p].nx, t.patches[p].ny, 64);
+        for (int k = 0; k < 8; ++k) nest_plan::add_gather_items(GI, t, k, 64);
+        CHECK(nest_plan::tables_in_bounds(t, I) && nest_plan::tables_in_bounds(t, GI), "batch bounds");
+        for (int k = 0; k < 8; ++k) CHECK(t.patches[k].prob == k && t.geoms[k].patch_base == k, "batch indices");
+        check_coverage(t, I, false, "batch patch strips");
+        check_coverage(t, GI, true, "batch global strips");
+        std::vector<double> a(t.total), b;
         std::mt19937_64 g(7);
         std::uniform_real_distribution<double> U(-1.0, 1.0);
         for (auto& x : a) x = U(g);
         b = a;
-        nest_plan::execute_regions_host(a.data(), R, 4);
+        nest_plan::execute_regions_host(a.data(), t.regions, 4);
         for (int k = 0; k < 8; ++k) {
             nest_oracle::mesh2d m(sizes[k], sizes[k] + 1, 1, 1, 2, 1, 1);
-            nest_oracle::fill_mesh(b.data() + P[k].off, m);
+            nest_oracle::fill_mesh(b.data() + t.patches[k].off, m);
         }
         CHECK(a == b, "batch fill");
     }
     // full-size configurations: bounds only
     for (auto [n, np] : {std::pair{4096, 16}, std::pair{8192, 64}, std::pair{1024, 1}}) {
-        std::vector<nest_patch_desc> P;
-        std::vector<nest_region> R;
-        std::vector<nest_item> I;
-        long long total = 0;
-        nest_plan::build_mesh(n, n, np, np, 1, 1, 0, P, R, total);
-        for (int p = 0; p < (int)P.size(); ++p) nest_plan::add_items(I, p, P[p].nx, P[p].ny, 64);
-        CHECK(nest_plan::tables_in_bounds(P, R, I, total), "full size %d", n);
+        tables t;
+        nest_plan::build_mesh(t, n, n, np, np, 1, 1);
+        std::vector<nest_item> I, GI;
+        for (int p = 0; p < (int)t.patches.size(); ++p)
+            nest_plan::add_patch_items(I, p, t.patches[p].nx, t.patches[p].ny, 64);
+        nest_plan::add_gather_items(GI, t, 0, 64);
+        CHECK(nest_plan::tables_in_bounds(t, I) && nest_plan::tables_in_bounds(t, GI), "full size %d", n);
     }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;
