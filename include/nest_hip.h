@@ -125,6 +125,11 @@ int nest_hip_sim2d_device_ptrs(nest_sim2d* s, double** dev_U, double** dev_U1);
    0 keeps the current value. persistent != 0 caps the grid at the number of
    resident waves and loops over the work list. */
 int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int persistent);
+/* Tiling of one fused stage (1 or 2) of the device-dt / rk2_step path only;
+   0 keeps the current value. get_tiling(): rows[2], waves_per_block[2] of
+   the two fused stages (either pointer may be NULL). */
+int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int waves_per_block);
+int nest_hip_sim2d_get_tiling(const nest_sim2d* s, int* rows, int* waves_per_block);
 /* host <-> device copies of the whole concatenated buffer U (blocking) */
 int nest_hip_sim2d_upload(nest_sim2d* s, const double* host);
 int nest_hip_sim2d_download(nest_sim2d* s, double* host);

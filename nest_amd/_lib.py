@@ -65,6 +65,8 @@ SIGNATURES = {
     "nest_hip_sim2d_bind": (_i, [_vp, _vp, _vp]),
     "nest_hip_sim2d_device_ptrs": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "nest_hip_sim2d_set_tiling": (_i, [_vp, _i, _i, _i]),
+    "nest_hip_sim2d_set_stage_tiling": (_i, [_vp, _i, _i, _i]),
+    "nest_hip_sim2d_get_tiling": (_i, [_vp, _pi, _pi]),
     "nest_hip_sim2d_upload": (_i, [_vp, _pd]),
     "nest_hip_sim2d_download": (_i, [_vp, _pd]),
     "nest_hip_sim2d_upload_patch": (_i, [_vp, _i, _pd]),

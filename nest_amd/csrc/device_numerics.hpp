@@ -26,9 +26,17 @@ enum : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_NONE = 2 };
 
 using eos_c = ::nest_eos;  // gamma, gm1 = gamma - 1.0, inv_gm1 = 1.0 / gm1 (fast mode only)
 
-// std::min / std::max semantics (selects on a strict compare)
+// std::min / std::max semantics (selects on a strict compare). Fast mode uses
+// the hardware v_min_f64 / v_max_f64, which agree except for the sign of a
+// zero result when the operands are zeros of opposite sign (every use below
+// only compares such a result against 0 or takes it where it cannot be a zero).
+#if NEST_FAST
+__device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
+#else
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+#endif
 
 #if NEST_FAST
 // 1/x: hardware estimate + two Newton-Raphson steps (error ~1 ulp)
@@ -49,8 +57,15 @@ __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 // ---------------------------------------------------------------------------
 // limiters — euler1d.hpp:112-137
 // ---------------------------------------------------------------------------
+// For a*b > 0 the reference's (|a| < |b|) ? a : b is the smaller magnitude
+// with the common sign: copysign(min(|a|, |b|), a) returns the same bits
+// (equal magnitudes give b's bits, as the reference does).
 __device__ __forceinline__ double minmod(double a, double b) {
+#if NEST_FAST
+    double r = __builtin_copysign(__builtin_fmin(__builtin_fabs(a), __builtin_fabs(b)), a);
+#else
     double r = (__builtin_fabs(a) < __builtin_fabs(b)) ? a : b;
+#endif
     return (a * b <= 0.0) ? 0.0 : r;
 }
 template <int LIM>

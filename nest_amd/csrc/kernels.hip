@@ -23,6 +23,14 @@
 #include "device_numerics.hpp"
 #include "kernels.hpp"
 
+// Minimum waves per SIMD requested for the stage kernel (register budget
+// 512/N per lane); unset = the compiler's choice.
+#ifdef NEST_STAGE_WAVES
+#define NEST_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NEST_STAGE_WAVES, 8)))
+#else
+#define NEST_STAGE_OCCUPANCY
+#endif
+
 namespace NEST_NS {
 
 __device__ __forceinline__ double ld(const double* p) { return *p; }
@@ -56,7 +64,7 @@ __device__ __forceinline__ void cell_speeds2d(double r, double mu, double mv, do
 }
 
 template <int LIM, int STAGE>
-__global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
+__global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
@@ -125,51 +133,71 @@ __global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
             u[3] = ld(p + 3 * (size_t)s);
         };
 
-        // ---- X window warm-up: rows i = r0-2 .. r0+1 (extended rows r0 .. r0+3)
-        double A[4], B[4], Ua[4], Ub[4];
-        load_row(w.r0, A);
-        load_row(w.r0 + 1, B);
-        load_row(w.r0 + 2, Ua);
-        load_row(w.r0 + 3, Ub);
-        prim4 Wm2 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
-        prim4 Wm1 = prim2d(B[0], B[1], B[2], B[3], true, a.eos);
-        prim4 W0 = prim2d(Ua[0], Ua[1], Ua[2], Ua[3], true, a.eos);
-        prim4 W1 = prim2d(Ub[0], Ub[1], Ub[2], Ub[3], true, a.eos);
-        prim4 dm1 = sub4(Wm1, Wm2);
-        prim4 d0 = sub4(W0, Wm1);
-        prim4 dW1 = sub4(W1, W0);
-        prim4 s_m1 = slope4<LIM>(dm1, d0);
-        prim4 s0 = slope4<LIM>(d0, dW1);
+        // ---- X window warm-up: rows i = r0-2 .. r0+1 (extended rows r0 .. r0+3).
+        // Carried down the march (registers): W1 = W(i+1), dW1 = W(i+1) - W(i),
+        // FL = W(i) + 0.5*slope(i) (left state of face i+1/2) and Fm = F(i-1/2).
+        prim4 W1, dW1, FL;
         double Fm[4];
-        face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+        {
+            double A[4];
+            load_row(w.r0, A);
+            prim4 Wm2 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            load_row(w.r0 + 1, A);
+            prim4 Wm1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            load_row(w.r0 + 2, A);
+            prim4 W0 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            load_row(w.r0 + 3, A);
+            W1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            prim4 d0 = sub4(W0, Wm1);
+            dW1 = sub4(W1, W0);
+            prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
+            prim4 s0 = slope4<LIM>(d0, dW1);
+            face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+            FL = face_plus(W0, s0);
+        }
 
         unsigned long long smx = 0, smy = 0;
         double Un[4];
         load_row(w.r0 + 4, Un);
-#pragma unroll 2
         for (int i = w.r0; i < w.r1; ++i) {
-            // prefetch the row the next iteration needs (clamped in bounds on the last)
-            double Unx[4];
+            // prefetch the row the next iteration needs (clamped in bounds on the
+            // last), and re-read this row's conserved state (cache-resident: it
+            // was loaded two iterations ago) rather than carrying it
+            double Unx[4], Uc[4];
             load_row(min(i + 5, pd.nx + 3), Unx);
+            {
+                const double* pc = base + (size_t)(i + 2) * snye;
+                Uc[0] = ld(pc);
+                Uc[1] = ld(pc + svs);
+                Uc[2] = ld(pc + 2 * (size_t)svs);
+                Uc[3] = ld(pc + 3 * (size_t)svs);
+            }
+            double* po = out + (size_t)(i + 2) * snye;
+            double U0[4];  // U^n at the output cell (stage 2 combine), issued early
+            if constexpr (STAGE == 2) {
+                if (store) {
+                    U0[0] = po[0];
+                    U0[1] = po[vs];
+                    U0[2] = po[2 * (size_t)vs];
+                    U0[3] = po[3 * (size_t)vs];
+                }
+            }
             prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
             prim4 dW2 = sub4(W2, W1);
             prim4 s1 = slope4<LIM>(dW1, dW2);
             double Fp[4];
-            face_flux<true>(face_plus(W0, s0), face_minus(W1, s1), a.eos, Fp);
+            face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
+            FL = face_plus(W1, s1);
             // x_sweep_t update (euler2d.hpp:144-148)
-            const double ux0 = Ua[0] - dtdx * (Fp[0] - Fm[0]);
-            const double ux1 = Ua[1] - dtdx * (Fp[1] - Fm[1]);
-            const double ux2 = Ua[2] - dtdx * (Fp[3] - Fm[3]);
-            const double ux3 = Ua[3] - dtdx * (Fp[2] - Fm[2]);
-            W0 = W1;
+            const double ux0 = Uc[0] - dtdx * (Fp[0] - Fm[0]);
+            const double ux1 = Uc[1] - dtdx * (Fp[1] - Fm[1]);
+            const double ux2 = Uc[2] - dtdx * (Fp[3] - Fm[3]);
+            const double ux3 = Uc[3] - dtdx * (Fp[2] - Fm[2]);
             W1 = W2;
             dW1 = dW2;
-            s0 = s1;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 Fm[k] = Fp[k];
-                Ua[k] = Ub[k];
-                Ub[k] = Un[k];
                 Un[k] = Unx[k];
             }
 
@@ -188,14 +216,13 @@ __global__ __launch_bounds__(256) void stage2d_kernel(stage_args a) {
             double o1 = ux1 - dtdy * (G[3] - Gm[3]);
             double o2 = ux2 - dtdy * (G[1] - Gm[1]);
             double o3 = ux3 - dtdy * (G[2] - Gm[2]);
-            double* po = out + (size_t)(i + 2) * snye;
             if constexpr (STAGE == 2) {
                 // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2), in place of U^n
                 if (store) {
-                    o0 = 0.5 * (po[0] + o0);
-                    o1 = 0.5 * (po[vs] + o1);
-                    o2 = 0.5 * (po[2 * (size_t)vs] + o2);
-                    o3 = 0.5 * (po[3 * (size_t)vs] + o3);
+                    o0 = 0.5 * (U0[0] + o0);
+                    o1 = 0.5 * (U0[1] + o1);
+                    o2 = 0.5 * (U0[2] + o2);
+                    o3 = 0.5 * (U0[3] + o3);
                     double sx, sy2;
                     cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
                     unsigned long long bx = (unsigned long long)__double_as_longlong(sx);

@@ -111,7 +111,7 @@ struct nest_sim2d {
     std::vector<nest_region>& regions_h = tb.regions;
     std::vector<nest_item> items_h;         // per-patch strips (own ghosts)
     std::vector<int> item_begin, item_end;  // per patch, into items_h
-    std::vector<nest_item> gitems_h;        // global strips (halo gathered from neighbours)
+    std::vector<nest_item> gitems_h[2];     // global strips per stage (halo gathered from neighbours)
     size_t total = 0;
 
     double* U_own = nullptr;
@@ -122,7 +122,7 @@ struct nest_sim2d {
     nest_region* regions_d = nullptr;
     nest_item* items_d = nullptr;
     nest_prob_state* probs_d = nullptr;
-    nest_item* gitems_d = nullptr;
+    nest_item* gitems_d[2] = {nullptr, nullptr};
     nest_prob_geom* geoms_d = nullptr;
     int* itab_d = nullptr;
 
@@ -131,8 +131,10 @@ struct nest_sim2d {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 
-    int rows = 64;
+    int rows = 64;              // per-patch strips
     int wpb = 4;
+    int grows[2] = {32, 48};    // global strips, rows per item, stage 1 / stage 2
+    int gwpb[2] = {4, 4};       // waves per block, stage 1 / stage 2
     int persistent = 0;
     int resident_waves = 0;
 
@@ -147,11 +149,11 @@ struct nest_sim2d {
     void release() {
         drop_graph();
         for (void* p : {(void*)U_own, (void*)U1_own, (void*)patches_d, (void*)regions_d, (void*)items_d,
-                        (void*)probs_d, (void*)gitems_d, (void*)geoms_d, (void*)itab_d})
+                        (void*)probs_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)geoms_d, (void*)itab_d})
             if (p) (void)hipFree(p);
         U_own = U1_own = U = U1 = nullptr;
         patches_d = nullptr;
-        gitems_d = nullptr;
+        gitems_d[0] = gitems_d[1] = nullptr;
         geoms_d = nullptr;
         itab_d = nullptr;
         regions_d = nullptr;
@@ -163,15 +165,15 @@ struct nest_sim2d {
 
     const launch_table& L() const { return table(mode); }
 
-    // gather != 0: global strips (gitems) reading every cell, halo included,
-    // from the patch that owns it (no exchange needed before the stage);
-    // 0: per-patch strips (items) reading the patch's own ghost cells.
+    // gather = 1 or 2: that stage's global strips (gitems) reading every cell,
+    // halo included, from the patch that owns it (no exchange needed before
+    // the stage); 0: per-patch strips (items) reading the patch's own ghosts.
     stage_args args(double host_dt, int item0, int nitems, int gather = 0) const {
         stage_args a{};
         a.U = U;
         a.U1 = U1;
         a.patches = patches_d;
-        a.items = (gather ? gitems_d : items_d) + item0;
+        a.items = (gather ? gitems_d[gather - 1] : items_d) + item0;
         a.nitems = nitems;
         a.probs = probs_d;
         a.eos = eos;
@@ -181,9 +183,9 @@ struct nest_sim2d {
         a.itab = itab_d;
         return a;
     }
-    int grid_for(int nitems) const {
-        int blocks = (nitems + wpb - 1) / wpb;
-        if (persistent && resident_waves > 0) blocks = std::min(blocks, std::max(1, resident_waves / wpb));
+    int grid_for(int nitems, int waves) const {
+        int blocks = (nitems + waves - 1) / waves;
+        if (persistent && resident_waves > 0) blocks = std::min(blocks, std::max(1, resident_waves / waves));
         return std::max(blocks, 1);
     }
 
@@ -197,16 +199,22 @@ struct nest_sim2d {
             nest_plan::add_patch_items(items_h, (int)p, patches_h[p].nx, patches_h[p].ny, rows);
             item_end[p] = (int)items_h.size();
         }
-        gitems_h.clear();
-        for (int q = 0; q < (int)tb.geoms.size(); ++q) nest_plan::add_gather_items(gitems_h, tb, q, rows);
+        for (int st = 0; st < 2; ++st) {
+            gitems_h[st].clear();
+            for (int q = 0; q < (int)tb.geoms.size(); ++q) nest_plan::add_gather_items(gitems_h[st], tb, q, grows[st]);
+        }
         if (int rc = validate_tables()) return rc;
-        for (nest_item** d : {&items_d, &gitems_d})
+        for (nest_item** d : {&items_d, &gitems_d[0], &gitems_d[1]}) {
             if (*d) (void)hipFree(*d);
-        items_d = gitems_d = nullptr;
+            *d = nullptr;
+        }
         if (int rc = dev_alloc(&items_d, items_h.size())) return rc;
-        if (int rc = dev_alloc(&gitems_d, gitems_h.size())) return rc;
         NEST_HIP_TRY(hipMemcpy(items_d, items_h.data(), items_h.size() * sizeof(nest_item), hipMemcpyHostToDevice));
-        NEST_HIP_TRY(hipMemcpy(gitems_d, gitems_h.data(), gitems_h.size() * sizeof(nest_item), hipMemcpyHostToDevice));
+        for (int st = 0; st < 2; ++st) {
+            if (int rc = dev_alloc(&gitems_d[st], gitems_h[st].size())) return rc;
+            NEST_HIP_TRY(hipMemcpy(gitems_d[st], gitems_h[st].data(), gitems_h[st].size() * sizeof(nest_item),
+                                   hipMemcpyHostToDevice));
+        }
         drop_graph();
         return NEST_OK;
     }
@@ -214,7 +222,8 @@ struct nest_sim2d {
     // Every device table entry must address memory inside the buffers: checked
     // on the host before anything is uploaded or launched.
     int validate_tables() const {
-        if (!nest_plan::tables_in_bounds(tb, items_h) || !nest_plan::tables_in_bounds(tb, gitems_h) ||
+        if (!nest_plan::tables_in_bounds(tb, items_h) || !nest_plan::tables_in_bounds(tb, gitems_h[0]) ||
+            !nest_plan::tables_in_bounds(tb, gitems_h[1]) ||
             (long long)total != tb.total)
             return fail(NEST_ERR_INVALID, "internal: a patch / halo region / work item addresses memory "
                                           "outside the buffers");
@@ -255,21 +264,28 @@ struct nest_sim2d {
 
     // ---- one launch of each step component ----
     void fill(double* base) { L().fill_regions(base, regions_d, (int)regions_h.size(), 4, stream); }
-    void stage(int st, double host_dt, int item0, int nitems, int gather = 0) {
+    // per-patch strips [item0, item0 + nitems) with a host or device dt
+    void stage(int st, double host_dt, int item0, int nitems) {
         if (nitems <= 0) return;
-        L().stage2d(st, lim, args(host_dt, item0, nitems, gather), grid_for(nitems), 64 * wpb, stream);
+        L().stage2d(st, lim, args(host_dt, item0, nitems, 0), grid_for(nitems, wpb), 64 * wpb, stream);
+    }
+    // all global strips of stage st (halo gathered in the kernel)
+    void gstage(int st, double host_dt) {
+        const int n = (int)gitems_h[st - 1].size();
+        const int w = gwpb[st - 1];
+        L().stage2d(st, lim, args(host_dt, 0, n, st), grid_for(n, w), 64 * w, stream);
     }
     void speeds(int item0, int nitems) {
         if (nitems <= 0) return;
-        L().speeds2d(args(0.0, item0, nitems), grid_for(nitems), 64 * wpb, stream);
+        L().speeds2d(args(0.0, item0, nitems), grid_for(nitems, wpb), 64 * wpb, stream);
     }
     // One fused step: device dt, then the two stages gathering their halos
     // from the neighbours (no exchange launches). Ghost cells of U are left
     // stale; callers refresh them with fill(U) when they hand U out.
     void device_step() {
         L().finalize(probs_d, nprob, stream);
-        stage(1, 0.0, 0, (int)gitems_h.size(), 1);
-        stage(2, 0.0, 0, (int)gitems_h.size(), 1);
+        gstage(1, 0.0);
+        gstage(2, 0.0);
     }
 };
 
@@ -413,13 +429,27 @@ int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int 
     // the stage kernels are compiled with __launch_bounds__(256): at most 4 waves per block
     if (rows < 0 || waves_per_block < 0 || waves_per_block > 4)
         return fail(NEST_ERR_INVALID, "rows >= 1, 1 <= waves_per_block <= 4");
-    if (waves_per_block) s->wpb = waves_per_block;
+    if (waves_per_block) s->wpb = s->gwpb[0] = s->gwpb[1] = waves_per_block;
     s->persistent = persistent;
-    if (rows && rows != s->rows) {
-        s->rows = rows;
-        return s->build_items();
+    if (rows) s->rows = s->grows[0] = s->grows[1] = rows;
+    return s->build_items();
+}
+
+int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int waves_per_block) {
+    if (int rc = valid_sim(s)) return rc;
+    if ((stage != 1 && stage != 2) || rows < 0 || waves_per_block < 0 || waves_per_block > 4)
+        return fail(NEST_ERR_INVALID, "stage 1|2, rows >= 1, 1 <= waves_per_block <= 4");
+    if (waves_per_block) s->gwpb[stage - 1] = waves_per_block;
+    if (rows) s->grows[stage - 1] = rows;
+    return s->build_items();
+}
+
+int nest_hip_sim2d_get_tiling(const nest_sim2d* s, int* rows, int* waves_per_block) {
+    if (int rc = valid_sim(s)) return rc;
+    for (int st = 0; st < 2; ++st) {
+        if (rows) rows[st] = s->grows[st];
+        if (waves_per_block) waves_per_block[st] = s->gwpb[st];
     }
-    s->drop_graph();
     return NEST_OK;
 }
 
@@ -492,9 +522,8 @@ int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch) {
 int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt) {
     if (int rc = valid_sim(s)) return rc;
     if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
-    const int n = (int)s->gitems_h.size();
-    s->stage(1, dt, 0, n, 1);
-    s->stage(2, dt, 0, n, 1);
+    s->gstage(1, dt);
+    s->gstage(2, dt);
     s->fill(s->U);
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
@@ -593,15 +622,14 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) rc = fail(NEST_ERR_HIP, "hipEventCreate failed");
     if (!rc) {
-        const int n = (int)s->gitems_h.size();
         for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(), with events between
             hipEvent_t* e = &ev[4 * (size_t)k];
             (void)hipEventRecord(e[0], s->stream);
             s->L().finalize(s->probs_d, s->nprob, s->stream);
             (void)hipEventRecord(e[1], s->stream);
-            s->stage(1, 0.0, 0, n, 1);
+            s->gstage(1, 0.0);
             (void)hipEventRecord(e[2], s->stream);
-            s->stage(2, 0.0, 0, n, 1);
+            s->gstage(2, 0.0);
             (void)hipEventRecord(e[3], s->stream);
         }
         hipError_t e = hipStreamSynchronize(s->stream);

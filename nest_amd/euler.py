@@ -178,6 +178,15 @@ class Sim2D:
         check(self.L.nest_hip_sim2d_info(self.h, C.byref(total), C.byref(npatch), C.byref(nprob), C.byref(nitems)))
         self.nitems = nitems.value
 
+    def set_stage_tiling(self, stage: int, rows: int = 0, waves_per_block: int = 0) -> None:
+        check(self.L.nest_hip_sim2d_set_stage_tiling(self.h, stage, rows, waves_per_block))
+
+    def get_tiling(self):
+        """((rows stage 1, rows stage 2), (waves/block stage 1, stage 2)) of the fused stages."""
+        r, w = (C.c_int * 2)(), (C.c_int * 2)()
+        check(self.L.nest_hip_sim2d_get_tiling(self.h, r, w))
+        return (r[0], r[1]), (w[0], w[1])
+
     # ---- numerics ----
     def fill_halos(self) -> None:
         check(self.L.nest_hip_sim2d_fill_halos(self.h))
