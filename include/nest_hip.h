@@ -139,6 +139,9 @@ int nest_hip_sim2d_download_patch(nest_sim2d* s, int patch, double* host);
    from its neighbour or physical boundary (periodic wrap / outflow copy);
    one launch. Extends execute_all_exchanges (reference patch.hpp:153-190). */
 int nest_hip_sim2d_fill_halos(nest_sim2d* s);
+/* The same exchange on a chosen buffer: 0 = U, 1 = U1 (the first-stage
+   state, whose halo a per-patch stage 2 reads). */
+int nest_hip_sim2d_exchange(nest_sim2d* s, int buffer);
 /* compute_dt_2d per problem into dt_host[nprob] (blocking). */
 int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host);
 /* One split stage with a host dt for every problem:

@@ -72,6 +72,7 @@ SIGNATURES = {
     "nest_hip_sim2d_upload_patch": (_i, [_vp, _i, _pd]),
     "nest_hip_sim2d_download_patch": (_i, [_vp, _i, _pd]),
     "nest_hip_sim2d_fill_halos": (_i, [_vp]),
+    "nest_hip_sim2d_exchange": (_i, [_vp, _i]),
     "nest_hip_sim2d_compute_dt": (_i, [_vp, _d, _pd]),
     "nest_hip_sim2d_stage": (_i, [_vp, _i, _d, _i]),
     "nest_hip_sim2d_rk2_step": (_i, [_vp, _d]),

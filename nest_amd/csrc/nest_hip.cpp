@@ -496,6 +496,14 @@ int nest_hip_sim2d_fill_halos(nest_sim2d* s) {
     return NEST_OK;
 }
 
+int nest_hip_sim2d_exchange(nest_sim2d* s, int buffer) {
+    if (int rc = valid_sim(s)) return rc;
+    if (buffer != 0 && buffer != 1) return fail(NEST_ERR_INVALID, "buffer must be 0 (U) or 1 (U1)");
+    s->fill(buffer ? s->U1 : s->U);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
 int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host) {
     if (int rc = valid_sim(s)) return rc;
     if (!dt_host) return fail(NEST_ERR_INVALID, "dt_host is null");
