@@ -66,6 +66,42 @@ def cpu_baseline(G, budget_s: float):
                       f"state each step), {el:.1f} s wall on {threads} OpenMP threads"}
 
 
+def ensemble_leg(steps: int, mode: str):
+    """Config 3: 512 uneven four-quadrant problems, 100 RK2 steps with
+    per-problem device dt. Batched (one launch per stage over all problems,
+    hipGraph-replayed step, autotuned tiles) against the plain per-patch path
+    (host dt with a blocking read-back, one launch set per problem)."""
+    from nest_amd import autotune, problems as P
+
+    sim, sizes, U0 = P.ensemble_batch(512, seed=3, mode=mode)
+    cells = float(sum(a * b for a, b in sizes))
+    sim.start(cfl=0.4)
+    cfg = autotune.tune(sim, sizes, log=log)
+    sim.upload(U0)
+    sim.start(cfl=0.4)
+    sim.advance(2, graph=True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(steps, graph=True)
+    sim.synchronize()
+    tb = time.perf_counter() - t0
+    pp_steps = max(5, steps // 10)
+    sim.upload(U0)
+    sim.advance_per_patch(1, 0.4)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.advance_per_patch(pp_steps, 0.4)
+    sim.synchronize()
+    tp = time.perf_counter() - t0
+    sim.close()
+    batched = cells * 2.0 * steps / tb / 1e6
+    per_patch = cells * 2.0 * pp_steps / tp / 1e6
+    return {"problems": 512, "cells": int(cells), "steps": steps, "batched_mcups": round(batched, 1),
+            "per_patch_mcups": round(per_patch, 1), "speedup": round(batched / per_patch, 2),
+            "per_patch_steps_timed": pp_steps,
+            "tiling": {k: v for k, v in cfg.items() if k in ("1", "2")}}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic_stage2d_sod4096.json")
     if os.path.exists(path):
@@ -85,6 +121,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--per-patch-steps", type=int, default=3)
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="config-3 leg; 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU tier); "
@@ -150,6 +187,11 @@ def main():
     sim.close()
     del U
 
+    ens = None
+    if args.ensemble_steps > 0:
+        log("bench: ensemble (512 problems)")
+        ens = ensemble_leg(args.ensemble_steps, args.mode)
+
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -178,6 +220,7 @@ def main():
                       "finalize": round(comp_ms[0] / args.steps, 4)},
         "graph_value": round(graph_value, 1),
         "per_patch_value": pp,
+        "ensemble": ens,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (traffic or {}).get("bytes_per_launch")},

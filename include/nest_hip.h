@@ -129,6 +129,10 @@ int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int 
    0 keeps the current value. get_tiling(): rows[2], waves_per_block[2] of
    the two fused stages (either pointer may be NULL). */
 int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int waves_per_block);
+/* Per-problem rows per work item of one fused stage (array of nprob; 0 =
+   the stage default; NULL clears): lets a tuner pick the tile height per
+   problem size class inside one launch. */
+int nest_hip_sim2d_set_problem_rows(nest_sim2d* s, int stage, const int* rows);
 int nest_hip_sim2d_get_tiling(const nest_sim2d* s, int* rows, int* waves_per_block);
 /* host <-> device copies of the whole concatenated buffer U (blocking) */
 int nest_hip_sim2d_upload(nest_sim2d* s, const double* host);

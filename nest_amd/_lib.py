@@ -67,6 +67,7 @@ SIGNATURES = {
     "nest_hip_sim2d_set_tiling": (_i, [_vp, _i, _i, _i]),
     "nest_hip_sim2d_set_stage_tiling": (_i, [_vp, _i, _i, _i]),
     "nest_hip_sim2d_get_tiling": (_i, [_vp, _pi, _pi]),
+    "nest_hip_sim2d_set_problem_rows": (_i, [_vp, _i, _pi]),
     "nest_hip_sim2d_upload": (_i, [_vp, _pd]),
     "nest_hip_sim2d_download": (_i, [_vp, _pd]),
     "nest_hip_sim2d_upload_patch": (_i, [_vp, _i, _pd]),

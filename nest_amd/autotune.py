@@ -1,0 +1,111 @@
+"""Autotuner of the fused stage kernels' work decomposition.
+
+Per stage (1 and 2) it picks, by measurement on the live problem set:
+  * rows per work item for each problem size class (the tile height of the
+    strip march: short tiles give more waves and a finer tail, long tiles
+    amortise the 4-row warm-up of the X window), and
+  * waves per block of the launch.
+Search: coordinate descent from the defaults, each candidate timed over a
+few device-dt steps with HIP events on the handle's stream (the same timing
+path as bench.py). Results are cached in a JSON file keyed by device name
+and the problem set's size-class histogram, so a later run on the same kind
+of input starts from the tuned point.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+ROWS = (16, 24, 32, 48, 64, 96, 128)
+WAVES = (1, 2, 4)
+
+
+def size_classes(sizes, edges=(128, 256)):
+    """Class index of each problem from its largest side: < 128, < 256, >= 256."""
+    return [int(np.searchsorted(edges, max(s), side="right")) for s in sizes]
+
+
+def signature(sizes, classes, arch: str) -> str:
+    hist = np.bincount(classes, minlength=3).tolist()
+    cells = int(sum(a * b for a, b in sizes))
+    return f"{arch}|classes={hist}|cells={cells}"
+
+
+def _cache_path(path: str | None) -> str:
+    if path:
+        return path
+    base = os.environ.get("XDG_CACHE_HOME", os.path.join(os.path.expanduser("~"), ".cache"))
+    return os.path.join(base, "nest_amd", "tuning.json")
+
+
+def _time(sim, steps):
+    sim.advance(1)
+    ms, _ = sim.advance_timed(steps)
+    return ms[1] / steps, ms[2] / steps
+
+
+def apply(sim, classes, cfg) -> None:
+    """Set a tuned configuration {stage: {"rows": [per class], "waves": w}}."""
+    for st in (1, 2):
+        c = cfg[str(st)]
+        sim.set_stage_tiling(st, 0, c["waves"])
+        sim.set_problem_rows(st, [c["rows"][k] for k in classes])
+
+
+def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = None, log=None):
+    """Tune `sim` (already started: start() called) in place and return the
+    configuration; `sizes` = [(nx, ny)] per problem."""
+    classes = size_classes(sizes)
+    key = signature(sizes, classes, arch)
+    path = _cache_path(cache)
+    table = {}
+    if os.path.exists(path):
+        try:
+            table = json.load(open(path))
+        except (OSError, ValueError):
+            table = {}
+    if key in table:
+        apply(sim, classes, table[key])
+        return table[key]
+    ncls = max(classes) + 1
+    cfg = {"1": {"rows": [32] * ncls, "waves": 4}, "2": {"rows": [48] * ncls, "waves": 4}}
+    apply(sim, classes, cfg)
+    best = sum(_time(sim, steps))
+    trials = []
+    for st in ("1", "2"):
+        for k in range(ncls):
+            if k not in classes:
+                continue
+            for r in ROWS:
+                if r == cfg[st]["rows"][k]:
+                    continue
+                trial = json.loads(json.dumps(cfg))
+                trial[st]["rows"][k] = r
+                apply(sim, classes, trial)
+                t = sum(_time(sim, steps))
+                trials.append((st, k, r, t))
+                if t < best:
+                    best, cfg = t, trial
+        for w in WAVES:
+            if w == cfg[st]["waves"]:
+                continue
+            trial = json.loads(json.dumps(cfg))
+            trial[st]["waves"] = w
+            apply(sim, classes, trial)
+            t = sum(_time(sim, steps))
+            trials.append((st, "waves", w, t))
+            if t < best:
+                best, cfg = t, trial
+    apply(sim, classes, cfg)
+    cfg["ms_per_step"] = round(best, 5)
+    if log:
+        log(f"autotune {key}: {cfg} ({len(trials)} trials)")
+    table[key] = cfg
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        json.dump(table, open(path, "w"), indent=1, sort_keys=True)
+    except OSError:
+        pass
+    return cfg

@@ -12,12 +12,21 @@
 //     the CPU path are a few ulp per operation; the per-step tolerance is
 //     1e-12 relative.
 #pragma once
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#else
+// host compilation (CPU unit tests of the per-face math, exact mode only)
+#define __device__
+#define __forceinline__ inline
+#endif
 
 #include "nest_types.hpp"
 
 #ifndef NEST_FAST
 #error "define NEST_FAST to 0 or 1"
+#endif
+#if NEST_FAST && !defined(__HIPCC__)
+#error "fast numerics use gfx950 builtins: device compilation only"
 #endif
 
 namespace NEST_NS {

@@ -134,6 +134,7 @@ struct nest_sim2d {
     int rows = 64;              // per-patch strips
     int wpb = 4;
     int grows[2] = {32, 48};    // global strips, rows per item, stage 1 / stage 2
+    std::vector<int> prob_rows[2];  // optional per-problem override of grows (0 = default)
     int gwpb[2] = {4, 4};       // waves per block, stage 1 / stage 2
     int persistent = 0;
     int resident_waves = 0;
@@ -201,7 +202,10 @@ struct nest_sim2d {
         }
         for (int st = 0; st < 2; ++st) {
             gitems_h[st].clear();
-            for (int q = 0; q < (int)tb.geoms.size(); ++q) nest_plan::add_gather_items(gitems_h[st], tb, q, grows[st]);
+            for (int q = 0; q < (int)tb.geoms.size(); ++q) {
+                const int r = (int)prob_rows[st].size() == nprob && prob_rows[st][q] > 0 ? prob_rows[st][q] : grows[st];
+                nest_plan::add_gather_items(gitems_h[st], tb, q, r);
+            }
         }
         if (int rc = validate_tables()) return rc;
         for (nest_item** d : {&items_d, &gitems_d[0], &gitems_d[1]}) {
@@ -441,6 +445,19 @@ int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int wave
         return fail(NEST_ERR_INVALID, "stage 1|2, rows >= 1, 1 <= waves_per_block <= 4");
     if (waves_per_block) s->gwpb[stage - 1] = waves_per_block;
     if (rows) s->grows[stage - 1] = rows;
+    return s->build_items();
+}
+
+int nest_hip_sim2d_set_problem_rows(nest_sim2d* s, int stage, const int* rows) {
+    if (int rc = valid_sim(s)) return rc;
+    if (stage != 1 && stage != 2) return fail(NEST_ERR_INVALID, "stage must be 1 or 2");
+    if (!rows) {
+        s->prob_rows[stage - 1].clear();
+    } else {
+        for (int i = 0; i < s->nprob; ++i)
+            if (rows[i] < 0) return fail(NEST_ERR_INVALID, "rows must be >= 0");
+        s->prob_rows[stage - 1].assign(rows, rows + s->nprob);
+    }
     return s->build_items();
 }
 

@@ -181,6 +181,14 @@ class Sim2D:
     def set_stage_tiling(self, stage: int, rows: int = 0, waves_per_block: int = 0) -> None:
         check(self.L.nest_hip_sim2d_set_stage_tiling(self.h, stage, rows, waves_per_block))
 
+    def set_problem_rows(self, stage: int, rows) -> None:
+        """Rows per work item for each problem of a batch (None clears)."""
+        if rows is None:
+            check(self.L.nest_hip_sim2d_set_problem_rows(self.h, stage, None))
+            return
+        arr = (C.c_int * self.nprob)(*[int(r) for r in rows])
+        check(self.L.nest_hip_sim2d_set_problem_rows(self.h, stage, arr))
+
     def get_tiling(self):
         """((rows stage 1, rows stage 2), (waves/block stage 1, stage 2)) of the fused stages."""
         r, w = (C.c_int * 2)(), (C.c_int * 2)()

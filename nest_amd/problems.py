@@ -115,3 +115,20 @@ def kelvin_helmholtz(n: int, seed=5, gamma=1.4):
     v = 0.01 * np.sin(4 * np.pi * X) + 1e-4 * (2.0 * uniform01(seed, idx) - 1.0)
     p = np.full_like(rho, 2.5)
     return np.stack([rho, rho * u, rho * v, energy2d(rho, u, v, p, gamma)])
+
+
+def ensemble_batch(nprob=512, seed=3, mode="fast", limiter="minmod", Sim2D=None):
+    """Config-3 problem set: `nprob` independent four-quadrant Riemann problems
+    on s x s cells (s log-uniform in [64, 512]), outflow, all in one batch
+    handle. Returns (sim, sizes, U) with U the uploaded halo-filled state."""
+    if Sim2D is None:
+        from .euler import Sim2D
+    sides = ensemble_sizes(nprob, seed)
+    sizes = [(s, s) for s in sides]
+    sim = Sim2D.batch(sizes, mode=mode, limiter=limiter, bc="outflow")
+    U = np.zeros(sim.total)
+    for k, s in enumerate(sides):
+        sim.patch_view(U, k)[:, 2:-2, 2:-2] = quadrants(s, k, seed=seed)
+    sim.upload(U)
+    sim.fill_halos()
+    return sim, sizes, sim.download()

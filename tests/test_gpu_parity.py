@@ -286,3 +286,31 @@ def test_config2_full_size_one_step(oracle):
     got = sim.download()
     oracle.mesh_rk2_step(U, m, dt, 1.0 / n, 1.0 / n)
     assert np.array_equal(got, U)
+
+
+def test_config3_ensemble_batched_equals_per_patch_and_oracle(oracle):
+    """Config 3 at full size (512 problems, 64..512 cells a side): 3 batched
+    device-dt steps equal 3 steps of the plain per-patch path bitwise, and a
+    sample of problems equals the oracle run alone."""
+    from nest_amd import problems as P
+    print_case("config3 ensemble")
+    a, sizes, U0 = P.ensemble_batch(512, seed=3, mode="exact")
+    b = Sim2D.batch(sizes, mode="exact")
+    b.upload(U0)
+    a.start(cfl=0.4)
+    a.advance(3, graph=True)
+    b.advance_per_patch(3, 0.4)
+    ga, gb = a.download(), b.download()
+    st = a.status()
+    assert (st["steps"] == 3).all() and (st["done"] == 0).all()
+    for k in (0, 1, 77, 511):
+        va, vb = a.patch_view(ga, k)[:, 2:-2, 2:-2], b.patch_view(gb, k)[:, 2:-2, 2:-2]
+        assert np.array_equal(va, vb)
+        s = sizes[k][0]
+        m = MeshSpec(s, s, 1, 1)
+        u = a.patch_view(U0, k).ravel().copy()
+        oracle.mesh_run(u, m, 3, 1.0 / s, 1.0 / s, cfl=0.4)
+        assert np.array_equal(va, u.reshape(4, s + 4, s + 4)[:, 2:-2, 2:-2]), k
+    # every problem: identical between the two paths
+    assert np.array_equal(ga, gb) or all(
+        np.array_equal(a.patch_view(ga, k)[:, 2:-2, 2:-2], b.patch_view(gb, k)[:, 2:-2, 2:-2]) for k in range(512))
