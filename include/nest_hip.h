@@ -173,6 +173,42 @@ int nest_hip_sim2d_status(nest_sim2d* s, double* t, double* dt, int* steps, int*
    stage. nsteps steps. */
 int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl);
 int nest_hip_sim2d_synchronize(nest_sim2d* s);
+
+/* ---- discrete adjoint of fixed-dt rollouts ---------------------------------
+   No counterpart in the reference (it has no differentiation). These are the
+   rollout_fwd / rollout_adj entry points behind the torch.autograd.Function
+   (nest_amd/adjoint.py). The differentiated map is the build's RK2 step at a
+   fixed dt, U1 = S(U), U' = 0.5 (U + S(U1)), S = Y o fill o X o fill of
+   x_sweep_t / y_sweep_t (examples/sod2d/euler2d.hpp:67-239) with the halo
+   fill (periodic wrap / zero-gradient copy) included. Derivatives are those
+   of the branch the primal takes (limiter cases, Davis speeds, HLLE upwind or
+   subsonic branch, floors). State and cotangent buffers are device pointers
+   in the handle's patch layout (size = total from nest_hip_sim2d_info); only
+   interior cells are read or written. Everything is enqueued on the handle's
+   stream. */
+
+/* Doubles in the problem-major interior layout [problem][var][x][y]. */
+int nest_hip_sim2d_global_size(const nest_sim2d* s, long long* n);
+/* direction 0: dst (interior layout) <- src (patch layout);
+   direction 1: dst (patch layout, interior cells) <- src (interior layout). */
+int nest_hip_sim2d_pack(nest_sim2d* s, const double* src, double* dst, int direction);
+/* Work split of the two adjoint kernels (0 keeps a value; waves <= 4). */
+int nest_hip_sim2d_set_adjoint_tiling(nest_sim2d* s, int rows_y, int waves_y, int rows_x, int waves_x);
+/* out = S'(U)^T (g*G) + h*H for one split stage (H may be null, and may
+   alias out; out may not alias U or G). */
+int nest_hip_sim2d_adjoint_stage(nest_sim2d* s, const double* U, const double* G, double g, const double* H,
+                                 double h, double* out, double dt);
+/* Ubar <- (dU'/dU)^T Ubar for one RK2 step of size dt taken from state U0. */
+int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double dt);
+/* nsteps fixed-dt RK2 steps of the handle's state, storing a checkpoint every
+   `interval` steps (<= 0: ceil(sqrt(nsteps))) in device buffers owned by the
+   handle; the final state's halos are refilled. */
+int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval);
+/* Reverse of the last rollout: Ubar (in: cotangent of the final state, out:
+   cotangent of the initial state), recomputing each checkpoint interval's
+   states from its checkpoint. Memory: ceil(K/m) + m - 1 + 3 states. */
+int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar);
+int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval, double* dt);
 /* advance() without a graph, with HIP events recorded on the handle's stream
    around each launch of the step; blocks at the end. ms[3] receives the
    summed durations of [dt finalize, stage 1, stage 2] over the nsteps steps,

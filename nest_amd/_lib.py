@@ -83,6 +83,14 @@ SIGNATURES = {
     "nest_hip_sim2d_advance_per_patch": (_i, [_vp, _i, _d]),
     "nest_hip_sim2d_synchronize": (_i, [_vp]),
     "nest_hip_sim2d_advance_timed": (_i, [_vp, _i, _pd, _pd]),
+    "nest_hip_sim2d_global_size": (_i, [_vp, C.POINTER(C.c_longlong)]),
+    "nest_hip_sim2d_pack": (_i, [_vp, _vp, _vp, _i]),
+    "nest_hip_sim2d_set_adjoint_tiling": (_i, [_vp, _i, _i, _i, _i]),
+    "nest_hip_sim2d_adjoint_stage": (_i, [_vp, _vp, _vp, _d, _vp, _d, _vp, _d]),
+    "nest_hip_sim2d_rk2_vjp": (_i, [_vp, _vp, _vp, _d]),
+    "nest_hip_sim2d_rollout": (_i, [_vp, _i, _d, _i]),
+    "nest_hip_sim2d_rollout_adjoint": (_i, [_vp, _vp]),
+    "nest_hip_sim2d_rollout_info": (_i, [_vp, _pi, _pi, _pd]),
 }
 
 _lib = None

@@ -1,0 +1,370 @@
+// Discrete adjoint of one split stage of the 2D Euler step, S = Y o fill o
+// X o fill, for gfx950 (wave64, FP64). Compiled once per numerics mode, like
+// the forward kernels.
+//
+// The halo fill is part of the differentiated map: every ghost cell is a
+// copy of the interior cell the boundary condition maps it to (periodic wrap
+// or zero-gradient clamp), so along each grid line a sweep is a function of
+// the interior cells only. Periodic lines are rings and need nothing more;
+// on an outflow line the two ghost positions past each end are copies of the
+// end cell, and the cotangents they receive are folded back into that cell.
+//
+// Y kernel (one wave per strip of 64 extended columns, lanes 3..60 own
+// output, marching interior rows like the forward stage kernel):
+//   * recomputes the X-swept row V(i) with the forward's sliding register
+//     window (the X-sweep of the halo columns is what the fill would copy);
+//   * Y-sweep reverse across lanes: cotangent of each y-face flux from the
+//     stage-output cotangent, reverse HLLE + passive flux, slopes, limiter
+//     and primitives, all neighbour terms moved by wave shuffles;
+//   * writes Vb(i) = g*G(i) + Y'(V(i))^T g*G(i) for the owned columns.
+// X kernel (one wave per strip of 64 columns, every lane owns its column;
+// the X-sweep is a per-column recurrence, so no lane exchange): marches
+// positions t along x with the primitives W(t-3..t) in registers; at each
+// step it reverses the x-face t-3/2 (its flux cotangent comes from Vb rows
+// t-2 and t-1), completes the slope cotangent of cell t-2 and reverses its
+// limiter, which completes the primitive cotangent of cell t-3; cell t-3 is
+// then emitted as out = Vb + (dW/dU)^T Wb (+ h*H).
+#include "adjoint_numerics.hpp"
+#include "kernels.hpp"
+
+namespace NEST_NS {
+
+__device__ __forceinline__ prim4 shup4(const prim4& a, int d) {
+    return {__shfl_up(a.r, d), __shfl_up(a.u, d), __shfl_up(a.p, d), __shfl_up(a.q, d)};
+}
+__device__ __forceinline__ prim4 shdn4(const prim4& a, int d) {
+    return {__shfl_down(a.r, d), __shfl_down(a.u, d), __shfl_down(a.p, d), __shfl_down(a.q, d)};
+}
+
+// ---------------------------------------------------------------------------
+// Y kernel
+// ---------------------------------------------------------------------------
+template <int LIM>
+__global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+        const nest_item w = a.items[it];
+        const nest_patch_desc pd = a.patches[w.patch];
+        const nest_prob_state& ps = a.probs[pd.prob];
+        const double dtdx = a.dt / ps.dx;
+        const double dtdy = a.dt / ps.dy;
+        const nest_prob_geom G = a.geoms[pd.prob];
+        // lane -> global column (unmapped position along y) -> owning patch
+        const int gs = pd.y0 + w.c0;
+        const int gcol = gs + lane - 3;
+        int q, le, lc;
+        nest_gather_source(pd, G, a.itab, 2, gcol, q, le, lc);
+        const nest_patch_desc qd = a.patches[q];
+        const int snye = qd.ny + 4;
+        const int svs = (qd.nx + 4) * snye;
+        const long long col = qd.off + lc;  // this lane's column, extended row 0
+        const bool ring = G.bcy == 0;
+        const bool real = ring || (gcol >= 0 && gcol < G.NY);  // outflow ghosts are not outputs
+        const bool own = lane >= 3 && lane < 61 && gcol < G.NY;
+        const bool edge = !ring && (gs == 0 || gs + 58 >= G.NY);  // wave-uniform
+
+        auto load_row = [&](int e, double* u) {
+            const double* p;
+            int s;
+            if (e < 2 || e >= pd.nx + 2) {  // ghost rows: wave-uniform slow path
+                int q2, le2, lc2;
+                nest_gather_source(pd, G, a.itab, e, gcol, q2, le2, lc2);
+                const nest_patch_desc qd2 = a.patches[q2];
+                const int qnye = qd2.ny + 4;
+                s = (qd2.nx + 4) * qnye;
+                p = a.U + qd2.off + (long long)le2 * qnye + lc2;
+            } else {
+                p = a.U + col + (long long)e * snye;
+                s = svs;
+            }
+            u[0] = p[0];
+            u[1] = p[s];
+            u[2] = p[2 * (long long)s];
+            u[3] = p[3 * (long long)s];
+        };
+
+        // X window warm-up (as the forward stage kernel)
+        prim4 W1, dW1, FL;
+        double Fm[4];
+        {
+            double A[4];
+            load_row(w.r0, A);
+            prim4 Wm2 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            load_row(w.r0 + 1, A);
+            prim4 Wm1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            load_row(w.r0 + 2, A);
+            prim4 W0 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            load_row(w.r0 + 3, A);
+            W1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            prim4 d0 = sub4(W0, Wm1);
+            dW1 = sub4(W1, W0);
+            prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
+            prim4 s0 = slope4<LIM>(d0, dW1);
+            face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+            FL = face_plus(W0, s0);
+        }
+        double Un[4];
+        load_row(w.r0 + 4, Un);
+        for (int i = w.r0; i < w.r1; ++i) {
+            double Unx[4], Uc[4];
+            load_row(min(i + 5, pd.nx + 3), Unx);
+            load_row(i + 2, Uc);
+            const double* pg = a.G + col + (long long)(i + 2) * snye;
+            double A[4];
+            A[0] = pg[0];
+            A[1] = pg[svs];
+            A[2] = pg[2 * (long long)svs];
+            A[3] = pg[3 * (long long)svs];
+            // ---- X-swept row (forward recompute)
+            prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
+            prim4 dW2 = sub4(W2, W1);
+            prim4 s1 = slope4<LIM>(dW1, dW2);
+            double Fp[4];
+            face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
+            FL = face_plus(W1, s1);
+            const double v0 = Uc[0] - dtdx * (Fp[0] - Fm[0]);
+            const double v1 = Uc[1] - dtdx * (Fp[1] - Fm[1]);
+            const double v2 = Uc[2] - dtdx * (Fp[3] - Fm[3]);
+            const double v3 = Uc[3] - dtdx * (Fp[2] - Fm[2]);
+            W1 = W2;
+            dW1 = dW2;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                Fm[k] = Fp[k];
+                Un[k] = Unx[k];
+            }
+            // ---- Y-sweep reverse across lanes
+#pragma unroll
+            for (int k = 0; k < 4; ++k) A[k] = real ? a.g * A[k] : 0.0;
+            const prim4 Y = prim2d(v0, v1, v2, v3, false, a.eos);
+            const prim4 dl = sub4(Y, shup4(Y, 1));
+            const prim4 dr = shdn4(dl, 1);
+            const prim4 sy = slope4<LIM>(dl, dr);
+            const coef4 cf = limiter_coef4<LIM>(dl, dr);
+            const prim4 Lf = face_plus(Y, sy);
+            const prim4 Rf = shdn4(face_minus(Y, sy), 1);
+            // face between this lane and the next: flux cotangent from the
+            // output cotangents of the two cells (y_sweep_t update order:
+            // rho <- G0, m_u <- G3, m_v <- G1, E <- G2)
+            double An[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) An[k] = __shfl_down(A[k], 1);
+            double Fb[4];
+            Fb[0] = dtdy * (An[0] - A[0]);
+            Fb[1] = dtdy * (An[2] - A[2]);
+            Fb[2] = dtdy * (An[3] - A[3]);
+            Fb[3] = dtdy * (An[1] - A[1]);
+            prim4 Lb = zero4(), Rb = zero4();
+            if (lane >= 1 && lane <= 61) face_flux_rev(Lf, Rf, a.eos, Fb, Lb, Rb);
+            const prim4 Rbm = shup4(Rb, 1);  // right-state cotangent of the face below this cell
+            const prim4 sb = scale4(0.5, sub4(Lb, Rbm));
+            const prim4 dlb = mul4(cf.l, sb);
+            const prim4 drb = mul4(cf.r, sb);
+            prim4 Wb = add4(add4(Lb, Rbm), sub4(dlb, drb));
+            Wb = sub4(Wb, shdn4(dlb, 1));
+            Wb = add4(Wb, shup4(drb, 1));
+            if (edge) {
+                // outflow: ghost positions -1, -2 are copies of column 0 and
+                // NY, NY+1 of column NY-1 (same primitives, so the
+                // cotangents add before the primitive reverse)
+                const prim4 u1 = shup4(Wb, 1), u2 = shup4(Wb, 2);
+                const prim4 d1 = shdn4(Wb, 1), d2 = shdn4(Wb, 2);
+                if (gcol == 0) Wb = add4(Wb, add4(u1, u2));
+                if (gcol == G.NY - 1) Wb = add4(Wb, add4(d1, d2));
+            }
+            double Ub[4];
+            prim2d_rev(Y, false, Wb, a.eos, Ub);
+            if (own) {
+                double* po = a.Vb + col + (long long)(i + 2) * snye;
+                po[0] = A[0] + Ub[0];
+                po[svs] = A[1] + Ub[1];
+                po[2 * (long long)svs] = A[2] + Ub[2];
+                po[3 * (long long)svs] = A[3] + Ub[3];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// X kernel
+// ---------------------------------------------------------------------------
+template <int LIM>
+__global__ __launch_bounds__(256) void adj_x_kernel(adj_args a) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+        const nest_item w = a.items[it];
+        const nest_patch_desc pd = a.patches[w.patch];
+        const nest_prob_state& ps = a.probs[pd.prob];
+        const double dtdx = a.dt / ps.dx;
+        const nest_prob_geom G = a.geoms[pd.prob];
+        const int gcol = pd.y0 + w.c0 + lane;
+        const bool own = gcol < G.NY;
+        const int gc = own ? gcol : G.NY - 1;  // lanes past the domain: any in-bounds column
+        int q, le, lc;
+        nest_gather_source(pd, G, a.itab, 2, gc, q, le, lc);
+        const nest_patch_desc qd = a.patches[q];
+        const int snye = qd.ny + 4;
+        const int svs = (qd.nx + 4) * snye;
+        const long long col = qd.off + lc;
+        const bool ring = G.bcx == 0;
+        const bool lo = !ring && pd.x0 + w.r0 == 0;     // outflow end at row 0
+        const bool hi = !ring && pd.x0 + w.r1 == G.NX;  // outflow end at row NX-1
+
+        // value of buffer B at position t (local row of the home patch, any
+        // integer; rows outside the patch resolve through the boundary map)
+        auto load_pos = [&](const double* B, int t, double* u) {
+            const int e = t + 2;
+            const double* p;
+            int s;
+            if (e < 2 || e >= pd.nx + 2) {
+                int q2, le2, lc2;
+                nest_gather_source(pd, G, a.itab, e, gc, q2, le2, lc2);
+                const nest_patch_desc qd2 = a.patches[q2];
+                const int qnye = qd2.ny + 4;
+                s = (qd2.nx + 4) * qnye;
+                p = B + qd2.off + (long long)le2 * qnye + lc2;
+            } else {
+                p = B + col + (long long)e * snye;
+                s = svs;
+            }
+            u[0] = p[0];
+            u[1] = p[s];
+            u[2] = p[2 * (long long)s];
+            u[3] = p[3 * (long long)s];
+        };
+        auto interior = [&](int t) {
+            const int gr = pd.x0 + t;
+            return ring || (gr >= 0 && gr < G.NX);
+        };
+        auto emit = [&](int row, const prim4& Wp, const prim4& Wbar, const double* Vbar) {
+            double Ub[4];
+            prim2d_rev(Wp, true, Wbar, a.eos, Ub);
+            const long long o = col + (long long)(row + 2) * snye;
+            double r[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = Vbar[k] + Ub[k];
+            if (a.H) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[k] += a.h * a.H[o + k * (long long)svs];
+            }
+            if (own) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) a.out[o + k * (long long)svs] = r[k];
+            }
+        };
+
+        const int tb = w.r0 - 3 - (lo ? 2 : 0);
+        const int te = w.r1 + 3 + (hi ? 2 : 0);
+        double u[4];
+        load_pos(a.U, tb, u);
+        prim4 W3 = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
+        prim4 W2 = W3, W1 = W3;
+        double Vb3[4] = {0.0, 0.0, 0.0, 0.0}, Vb2[4] = {0.0, 0.0, 0.0, 0.0};
+        prim4 Wb3 = zero4(), Wb2 = zero4(), sb2 = zero4(), acc = zero4();
+        double Vhold[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int t = tb + 1; t < te; ++t) {
+            load_pos(a.U, t, u);
+            const prim4 Wt = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
+            double Vb1[4] = {0.0, 0.0, 0.0, 0.0};
+            if (interior(t - 1)) load_pos(a.Vb, t - 1, Vb1);
+            const prim4 d3 = sub4(W2, W3), d2 = sub4(W1, W2), d1 = sub4(Wt, W1);
+            const prim4 s2 = slope4<LIM>(d3, d2);
+            const prim4 s1 = slope4<LIM>(d2, d1);
+            // face t-3/2 (x_sweep_t update order: rho <- F0, m_u <- F1, m_v <- F3, E <- F2)
+            double Fb[4];
+            Fb[0] = dtdx * (Vb1[0] - Vb2[0]);
+            Fb[1] = dtdx * (Vb1[1] - Vb2[1]);
+            Fb[2] = dtdx * (Vb1[3] - Vb2[3]);
+            Fb[3] = dtdx * (Vb1[2] - Vb2[2]);
+            prim4 Lb = zero4(), Rb = zero4();
+            face_flux_rev(face_plus(W2, s2), face_minus(W1, s1), a.eos, Fb, Lb, Rb);
+            Wb2 = add4(Wb2, Lb);
+            sb2 = add4(sb2, scale4(0.5, Lb));  // slope cotangent of cell t-2 complete
+            prim4 Wb1 = Rb;
+            const prim4 sb1 = scale4(-0.5, Rb);
+            const coef4 cf = limiter_coef4<LIM>(d3, d2);
+            const prim4 dlb = mul4(cf.l, sb2), drb = mul4(cf.r, sb2);
+            Wb2 = add4(Wb2, sub4(dlb, drb));
+            Wb3 = sub4(Wb3, dlb);  // primitive cotangent of cell t-3 complete
+            Wb1 = add4(Wb1, drb);
+
+            const int p = t - 3;
+            if (lo && p < 0) {
+                acc = add4(acc, Wb3);
+            } else if (lo && p == 0) {
+                emit(0, W3, add4(Wb3, acc), Vb3);
+                acc = zero4();
+            } else if (hi && p == w.r1 - 1) {
+                acc = Wb3;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Vhold[k] = Vb3[k];
+            } else if (hi && p >= w.r1) {
+                acc = add4(acc, Wb3);
+                if (p == w.r1 + 1) emit(w.r1 - 1, W3, acc, Vhold);  // W3: ghost copy of row NX-1
+            } else if (p >= w.r0 && p < w.r1) {
+                emit(p, W3, Wb3, Vb3);
+            }
+            W3 = W2;
+            W2 = W1;
+            W1 = Wt;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                Vb3[k] = Vb2[k];
+                Vb2[k] = Vb1[k];
+            }
+            Wb3 = Wb2;
+            Wb2 = Wb1;
+            sb2 = sb1;
+        }
+    }
+}
+
+// Copies between the patch layout and the problem-major interior layout
+// [problem][var][x][y] (problem q starts at goff[q]): dir 0 patch -> global,
+// dir 1 global -> patch (interior cells only; ghosts untouched).
+__global__ __launch_bounds__(256) void pack_kernel(const double* src, double* dst, const nest_patch_desc* P,
+                                                   const nest_prob_geom* geoms, const long long* goff, int dir) {
+    const nest_patch_desc d = P[blockIdx.x];
+    const int k = blockIdx.y;
+    const nest_prob_geom G = geoms[d.prob];
+    const int nye = d.ny + 4;
+    const long long vs = (long long)(d.nx + 4) * nye;
+    const long long gbase = goff[d.prob] + (long long)k * G.NX * G.NY;
+    const int n = d.nx * d.ny;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int ix = c / d.ny, iy = c - ix * d.ny;
+        const long long ps = d.off + k * vs + (long long)(ix + 2) * nye + iy + 2;
+        const long long gi = gbase + (long long)(d.x0 + ix) * G.NY + d.y0 + iy;
+        if (dir == 0)
+            dst[gi] = src[ps];
+        else
+            dst[ps] = src[gi];
+    }
+}
+
+void launch_pack(const double* src, double* dst, const nest_patch_desc* P, int npatch, const nest_prob_geom* geoms,
+                 const long long* goff, int dir, hipStream_t s) {
+    if (npatch <= 0) return;
+    hipLaunchKernelGGL(pack_kernel, dim3(npatch, 4), dim3(256), 0, s, src, dst, P, geoms, goff, dir);
+}
+
+void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s) {
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL(adj_y_kernel<LIM_MC>, grid, block, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL(adj_y_kernel<LIM_NONE>, grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL(adj_y_kernel<LIM_MINMOD>, grid, block, 0, s, a); break;
+    }
+}
+void launch_adj_x(int lim, const adj_args& a, int grid, int block, hipStream_t s) {
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL(adj_x_kernel<LIM_MC>, grid, block, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL(adj_x_kernel<LIM_NONE>, grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL(adj_x_kernel<LIM_MINMOD>, grid, block, 0, s, a); break;
+    }
+}
+
+}  // namespace NEST_NS

@@ -17,6 +17,10 @@
     void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s);                  \
     void launch_speeds1d(const stage1d_args& a, hipStream_t s);                                     \
     void launch_fill1d(double* U, int n, int bc, const nest_prob_state* ps, hipStream_t s);         \
+    void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s);              \
+    void launch_adj_x(int lim, const adj_args& a, int grid, int block, hipStream_t s);              \
+    void launch_pack(const double* src, double* dst, const nest_patch_desc* P, int npatch,          \
+                     const nest_prob_geom* geoms, const long long* goff, int dir, hipStream_t s);   \
     }
 
 // arguments of the 2D stage / speed kernels
@@ -33,6 +37,29 @@ struct stage_args {
     // read from the patch that owns it (nest_gather_source); 0: items are
     // per-patch strips reading the patch's own ghost cells.
     int gather;
+    const nest_prob_geom* geoms;
+    const int* itab;
+};
+
+// arguments of the two adjoint kernels of one split stage S = Y o X (the
+// halo fills included): the Y kernel maps the cotangent g*G of S(U) to the
+// cotangent Vb of the X-swept state; the X kernel maps Vb to
+// out = X'(U)^T Vb + h*H. All buffers are in the state's patch layout; only
+// interior cells are read or written.
+struct adj_args {
+    const double* U;  // stage input state
+    const double* G;  // Y kernel: cotangent of the stage output
+    double g;         // its scale
+    double* Vb;       // Y kernel output, X kernel input
+    const double* H;  // X kernel: added term (null: none); may alias out
+    double h;
+    double* out;      // X kernel output
+    const nest_patch_desc* patches;
+    const nest_item* items;
+    int nitems;
+    const nest_prob_state* probs;
+    nest_eos eos;
+    double dt;
     const nest_prob_geom* geoms;
     const int* itab;
 };

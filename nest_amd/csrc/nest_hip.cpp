@@ -43,16 +43,24 @@ struct launch_table {
     void (*stage1d)(int, int, const stage1d_args&, hipStream_t);
     void (*speeds1d)(const stage1d_args&, hipStream_t);
     void (*fill1d)(double*, int, int, const nest_prob_state*, hipStream_t);
+    void (*adj_y)(int, const adj_args&, int, int, hipStream_t);
+    void (*adj_x)(int, const adj_args&, int, int, hipStream_t);
+    void (*pack)(const double*, double*, const nest_patch_desc*, int, const nest_prob_geom*, const long long*, int,
+                 hipStream_t);
 };
 
 const launch_table kExact = {nest_exact::launch_stage2d,     nest_exact::launch_speeds2d,
                              nest_exact::launch_fill_regions, nest_exact::launch_finalize,
                              nest_exact::launch_reset_speeds, nest_exact::launch_stage1d,
-                             nest_exact::launch_speeds1d,     nest_exact::launch_fill1d};
+                             nest_exact::launch_speeds1d,     nest_exact::launch_fill1d,
+                             nest_exact::launch_adj_y,        nest_exact::launch_adj_x,
+                             nest_exact::launch_pack};
 const launch_table kFast = {nest_fast::launch_stage2d,     nest_fast::launch_speeds2d,
                             nest_fast::launch_fill_regions, nest_fast::launch_finalize,
                             nest_fast::launch_reset_speeds, nest_fast::launch_stage1d,
-                            nest_fast::launch_speeds1d,     nest_fast::launch_fill1d};
+                            nest_fast::launch_speeds1d,     nest_fast::launch_fill1d,
+                            nest_fast::launch_adj_y,        nest_fast::launch_adj_x,
+                            nest_fast::launch_pack};
 
 const launch_table& table(int mode) { return mode == NEST_MODE_FAST ? kFast : kExact; }
 
@@ -139,6 +147,22 @@ struct nest_sim2d {
     int persistent = 0;
     int resident_waves = 0;
 
+    // ---- discrete adjoint of fixed-dt rollouts
+    std::vector<nest_item> aitems_h[2];  // adjoint strips: Y kernel (58 owned columns), X kernel (64)
+    nest_item* aitems_d[2] = {nullptr, nullptr};
+    int arows[2] = {32, 32};
+    int awpb[2] = {4, 4};
+    double* adj_U1 = nullptr;   // stage-1 state of the step being reversed
+    double* adj_Vb = nullptr;   // cotangent of the X-swept intermediate
+    double* adj_Ub1 = nullptr;  // cotangent of the stage-1 state
+    std::vector<long long> goff_h;  // problem offsets in the problem-major interior layout
+    long long gtotal = 0;
+    long long* goff_d = nullptr;
+    std::vector<double*> ckpt;  // rollout checkpoints (every roll_m steps)
+    std::vector<double*> seg;   // recomputed states of one checkpoint interval
+    int roll_K = 0, roll_m = 0;
+    double roll_dt = 0.0;
+
     ~nest_sim2d() { release(); }
 
     void drop_graph() {
@@ -150,8 +174,18 @@ struct nest_sim2d {
     void release() {
         drop_graph();
         for (void* p : {(void*)U_own, (void*)U1_own, (void*)patches_d, (void*)regions_d, (void*)items_d,
-                        (void*)probs_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)geoms_d, (void*)itab_d})
+                        (void*)probs_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)geoms_d, (void*)itab_d,
+                        (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
+                        (void*)goff_d})
             if (p) (void)hipFree(p);
+        for (auto* v : {&ckpt, &seg}) {
+            for (double* p : *v) (void)hipFree(p);
+            v->clear();
+        }
+        aitems_d[0] = aitems_d[1] = nullptr;
+        adj_U1 = adj_Vb = adj_Ub1 = nullptr;
+        goff_d = nullptr;
+        roll_K = 0;
         U_own = U1_own = U = U1 = nullptr;
         patches_d = nullptr;
         gitems_d[0] = gitems_d[1] = nullptr;
@@ -207,8 +241,13 @@ struct nest_sim2d {
                 nest_plan::add_gather_items(gitems_h[st], tb, q, r);
             }
         }
+        for (int k = 0; k < 2; ++k) {
+            aitems_h[k].clear();
+            for (int q = 0; q < (int)tb.geoms.size(); ++q)
+                nest_plan::add_gather_items(aitems_h[k], tb, q, arows[k], k == 0 ? 58 : 64);
+        }
         if (int rc = validate_tables()) return rc;
-        for (nest_item** d : {&items_d, &gitems_d[0], &gitems_d[1]}) {
+        for (nest_item** d : {&items_d, &gitems_d[0], &gitems_d[1], &aitems_d[0], &aitems_d[1]}) {
             if (*d) (void)hipFree(*d);
             *d = nullptr;
         }
@@ -217,6 +256,9 @@ struct nest_sim2d {
         for (int st = 0; st < 2; ++st) {
             if (int rc = dev_alloc(&gitems_d[st], gitems_h[st].size())) return rc;
             NEST_HIP_TRY(hipMemcpy(gitems_d[st], gitems_h[st].data(), gitems_h[st].size() * sizeof(nest_item),
+                                   hipMemcpyHostToDevice));
+            if (int rc = dev_alloc(&aitems_d[st], aitems_h[st].size())) return rc;
+            NEST_HIP_TRY(hipMemcpy(aitems_d[st], aitems_h[st].data(), aitems_h[st].size() * sizeof(nest_item),
                                    hipMemcpyHostToDevice));
         }
         drop_graph();
@@ -227,7 +269,8 @@ struct nest_sim2d {
     // on the host before anything is uploaded or launched.
     int validate_tables() const {
         if (!nest_plan::tables_in_bounds(tb, items_h) || !nest_plan::tables_in_bounds(tb, gitems_h[0]) ||
-            !nest_plan::tables_in_bounds(tb, gitems_h[1]) ||
+            !nest_plan::tables_in_bounds(tb, gitems_h[1]) || !nest_plan::tables_in_bounds(tb, aitems_h[0]) ||
+            !nest_plan::tables_in_bounds(tb, aitems_h[1]) ||
             (long long)total != tb.total)
             return fail(NEST_ERR_INVALID, "internal: a patch / halo region / work item addresses memory "
                                           "outside the buffers");
@@ -257,6 +300,14 @@ struct nest_sim2d {
                                    hipMemcpyHostToDevice));
         NEST_HIP_TRY(hipMemcpy(probs_d, probs_h.data(), probs_h.size() * sizeof(nest_prob_state),
                                hipMemcpyHostToDevice));
+        goff_h.assign(tb.geoms.size(), 0);
+        gtotal = 0;
+        for (size_t q = 0; q < tb.geoms.size(); ++q) {
+            goff_h[q] = gtotal;
+            gtotal += 4LL * tb.geoms[q].NX * tb.geoms[q].NY;
+        }
+        if (int rc = dev_alloc(&goff_d, goff_h.size())) return rc;
+        NEST_HIP_TRY(hipMemcpy(goff_d, goff_h.data(), goff_h.size() * sizeof(long long), hipMemcpyHostToDevice));
         NEST_HIP_TRY(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
         stream = own_stream;
         int dev = 0, cus = 0;
@@ -290,6 +341,66 @@ struct nest_sim2d {
         L().finalize(probs_d, nprob, stream);
         gstage(1, 0.0);
         gstage(2, 0.0);
+    }
+
+    // ---- adjoint and fixed-dt rollouts ----
+    // forward stage st on an arbitrary state buffer X (stage 1: X -> X1;
+    // stage 2: X <- 0.5 (X + S(X1)) in place), host dt
+    void stage_on(int st, double* X, double* X1, double dt) {
+        const int n = (int)gitems_h[st - 1].size();
+        const int w = gwpb[st - 1];
+        stage_args a = args(dt, 0, n, st);
+        a.U = X;
+        a.U1 = X1;
+        L().stage2d(st, lim, a, grid_for(n, w), 64 * w, stream);
+    }
+    int ensure_adjoint_scratch() {
+        for (double** p : {&adj_U1, &adj_Vb, &adj_Ub1})
+            if (!*p) {
+                if (int rc = dev_alloc(p, total)) return rc;
+                NEST_HIP_TRY(hipMemset(*p, 0, total * sizeof(double)));
+            }
+        return NEST_OK;
+    }
+    int grow(std::vector<double*>& v, size_t n) {
+        while (v.size() < n) {
+            double* p = nullptr;
+            if (int rc = dev_alloc(&p, total)) return rc;
+            v.push_back(p);
+        }
+        return NEST_OK;
+    }
+    // out = S'(Uin)^T (g G) + h H for one split stage S = Y o fill o X o fill
+    void adj_stage(const double* Uin, const double* G, double g, const double* H, double h, double* out,
+                   double dt) {
+        adj_args a{};
+        a.U = Uin;
+        a.G = G;
+        a.g = g;
+        a.Vb = adj_Vb;
+        a.H = H;
+        a.h = h;
+        a.out = out;
+        a.patches = patches_d;
+        a.probs = probs_d;
+        a.eos = eos;
+        a.dt = dt;
+        a.geoms = geoms_d;
+        a.itab = itab_d;
+        for (int k = 0; k < 2; ++k) {
+            a.items = aitems_d[k];
+            a.nitems = (int)aitems_h[k].size();
+            const int w = awpb[k];
+            (k == 0 ? L().adj_y : L().adj_x)(lim, a, grid_for(a.nitems, w), 64 * w, stream);
+        }
+    }
+    // Ub <- (dU^{n+1}/dU^n)^T Ub for the fixed-dt RK2 step taken from U0:
+    // U1 = S(U0), U^{n+1} = 0.5 (U0 + S(U1)), so
+    // Ub1 = S'(U1)^T (0.5 Ub) and Ub <- 0.5 Ub + S'(U0)^T Ub1.
+    void vjp_step(const double* U0, double* Ub, double dt) {
+        stage_on(1, const_cast<double*>(U0), adj_U1, dt);
+        adj_stage(adj_U1, Ub, 0.5, nullptr, 0.0, adj_Ub1, dt);
+        adj_stage(U0, adj_Ub1, 1.0, Ub, 0.5, Ub, dt);
     }
 };
 
@@ -684,6 +795,112 @@ int nest_hip_sim2d_synchronize(nest_sim2d* s) {
     if (int rc = valid_sim(s)) return rc;
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
     NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+// ---- discrete adjoint ------------------------------------------------------
+
+int nest_hip_sim2d_global_size(const nest_sim2d* s, long long* n) {
+    if (int rc = valid_sim(s)) return rc;
+    if (!n) return fail(NEST_ERR_INVALID, "n is null");
+    *n = s->gtotal;
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_pack(nest_sim2d* s, const double* src, double* dst, int direction) {
+    if (int rc = valid_sim(s)) return rc;
+    if (!src || !dst) return fail(NEST_ERR_INVALID, "null buffer");
+    if (direction != 0 && direction != 1) return fail(NEST_ERR_INVALID, "direction must be 0 or 1");
+    s->L().pack(src, dst, s->patches_d, (int)s->patches_h.size(), s->geoms_d, s->goff_d, direction, s->stream);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_set_adjoint_tiling(nest_sim2d* s, int rows_y, int waves_y, int rows_x, int waves_x) {
+    if (int rc = valid_sim(s)) return rc;
+    if (rows_y < 0 || rows_x < 0 || waves_y < 0 || waves_y > 4 || waves_x < 0 || waves_x > 4)
+        return fail(NEST_ERR_INVALID, "rows >= 1, 1 <= waves_per_block <= 4");
+    if (rows_y) s->arows[0] = rows_y;
+    if (rows_x) s->arows[1] = rows_x;
+    if (waves_y) s->awpb[0] = waves_y;
+    if (waves_x) s->awpb[1] = waves_x;
+    return s->build_items();
+}
+
+int nest_hip_sim2d_adjoint_stage(nest_sim2d* s, const double* U, const double* G, double g, const double* H,
+                                 double h, double* out, double dt) {
+    if (int rc = valid_sim(s)) return rc;
+    if (!U || !G || !out) return fail(NEST_ERR_INVALID, "null buffer");
+    if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
+    if (out == G || out == U) return fail(NEST_ERR_INVALID, "out may alias H only");
+    if (int rc = s->ensure_adjoint_scratch()) return rc;
+    s->adj_stage(U, G, g, H, h, out, dt);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double dt) {
+    if (int rc = valid_sim(s)) return rc;
+    if (!U0 || !Ubar) return fail(NEST_ERR_INVALID, "null buffer");
+    if (U0 == Ubar) return fail(NEST_ERR_INVALID, "U0 and Ubar must differ");
+    if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
+    if (int rc = s->ensure_adjoint_scratch()) return rc;
+    s->vjp_step(U0, Ubar, dt);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
+    if (int rc = valid_sim(s)) return rc;
+    if (nsteps < 1 || !(dt > 0.0)) return fail(NEST_ERR_INVALID, "nsteps >= 1 and dt > 0");
+    const int m = interval > 0 ? interval : std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
+    const int nck = (nsteps + m - 1) / m;
+    if (int rc = s->grow(s->ckpt, (size_t)nck)) return rc;
+    for (int k = 0; k < nsteps; ++k) {
+        if (k % m == 0)
+            NEST_HIP_TRY(hipMemcpyAsync(s->ckpt[k / m], s->U, s->total * sizeof(double), hipMemcpyDeviceToDevice,
+                                        s->stream));
+        s->stage_on(1, s->U, s->U1, dt);
+        s->stage_on(2, s->U, s->U1, dt);
+    }
+    s->fill(s->U);
+    s->roll_K = nsteps;
+    s->roll_m = m;
+    s->roll_dt = dt;
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
+    if (int rc = valid_sim(s)) return rc;
+    if (!Ubar) return fail(NEST_ERR_INVALID, "Ubar is null");
+    if (s->roll_K < 1) return fail(NEST_ERR_INVALID, "no rollout recorded: call nest_hip_sim2d_rollout first");
+    if (int rc = s->ensure_adjoint_scratch()) return rc;
+    const int K = s->roll_K, m = s->roll_m;
+    const double dt = s->roll_dt;
+    if (int rc = s->grow(s->seg, (size_t)std::max(0, m - 1))) return rc;
+    const int nck = (K + m - 1) / m;
+    for (int c = nck - 1; c >= 0; --c) {
+        const int len = std::min(m, K - c * m);
+        // states of steps c*m .. c*m + len - 1: the checkpoint, then recomputed
+        auto state = [&](int i) { return i == 0 ? s->ckpt[c] : s->seg[i - 1]; };
+        for (int i = 1; i < len; ++i) {
+            NEST_HIP_TRY(hipMemcpyAsync(state(i), state(i - 1), s->total * sizeof(double),
+                                        hipMemcpyDeviceToDevice, s->stream));
+            s->stage_on(1, state(i), s->U1, dt);
+            s->stage_on(2, state(i), s->U1, dt);
+        }
+        for (int i = len - 1; i >= 0; --i) s->vjp_step(state(i), Ubar, dt);
+    }
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval, double* dt) {
+    if (int rc = valid_sim(s)) return rc;
+    if (nsteps) *nsteps = s->roll_K;
+    if (interval) *interval = s->roll_m;
+    if (dt) *dt = s->roll_dt;
     return NEST_OK;
 }
 

@@ -142,16 +142,17 @@ inline void add_patch_items(std::vector<nest_item>& items, int patch, int nx, in
 }
 
 // Global-strip work list of one problem (halo gathered from the neighbours):
-// for every dim-0 partition and row chunk, strips of 60 output columns over
+// for every dim-0 partition and row chunk, strips of `width` output columns
+// (60 for the forward stage kernel) over
 // the problem's whole dim-1 range, each anchored in the patch holding its
 // first output column. A strip's lanes may run into the next patches.
-inline void add_gather_items(std::vector<nest_item>& items, const tables& t, int prob, int rows) {
+inline void add_gather_items(std::vector<nest_item>& items, const tables& t, int prob, int rows, int width = 60) {
     const nest_prob_geom& G = t.geoms[prob];
     const int* T = t.itab.data() + G.tab;
     for (int px = 0; px < G.npx; ++px) {
         const int lx = t.patches[G.patch_base + px].nx;
         for (int r0 = 0; r0 < lx; r0 += rows)
-            for (int gs = 0; gs < G.NY; gs += 60) {
+            for (int gs = 0; gs < G.NY; gs += width) {
                 const int qy = T[G.NX + gs];
                 const int home = G.patch_base + qy * G.npx + px;
                 const int c0 = gs - T[G.NX + G.NY + G.npx + qy];

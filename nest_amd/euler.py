@@ -238,6 +238,47 @@ class Sim2D:
     def synchronize(self) -> None:
         check(self.L.nest_hip_sim2d_synchronize(self.h))
 
+    # ---- discrete adjoint (device pointers in the patch layout) ----
+    def global_size(self) -> int:
+        """Doubles in the problem-major interior layout [problem][var][x][y]."""
+        n = C.c_longlong()
+        check(self.L.nest_hip_sim2d_global_size(self.h, C.byref(n)))
+        return n.value
+
+    def pack(self, src: int, dst: int) -> None:
+        """dst (interior layout) <- src (patch layout), device pointers."""
+        check(self.L.nest_hip_sim2d_pack(self.h, C.c_void_p(src), C.c_void_p(dst), 0))
+
+    def unpack(self, src: int, dst: int) -> None:
+        """dst (patch layout, interior cells) <- src (interior layout)."""
+        check(self.L.nest_hip_sim2d_pack(self.h, C.c_void_p(src), C.c_void_p(dst), 1))
+
+    def set_adjoint_tiling(self, rows_y=0, waves_y=0, rows_x=0, waves_x=0) -> None:
+        check(self.L.nest_hip_sim2d_set_adjoint_tiling(self.h, rows_y, waves_y, rows_x, waves_x))
+
+    def adjoint_stage(self, U: int, G: int, g: float, H: int | None, h: float, out: int, dt: float) -> None:
+        """out = S'(U)^T (g G) + h H for one split stage."""
+        check(self.L.nest_hip_sim2d_adjoint_stage(self.h, C.c_void_p(U), C.c_void_p(G), g, C.c_void_p(H or 0), h,
+                                                  C.c_void_p(out), dt))
+
+    def rk2_vjp(self, U0: int, Ubar: int, dt: float) -> None:
+        """Ubar <- (dU'/dU)^T Ubar for one fixed-dt RK2 step from state U0."""
+        check(self.L.nest_hip_sim2d_rk2_vjp(self.h, C.c_void_p(U0), C.c_void_p(Ubar), dt))
+
+    def rollout(self, nsteps: int, dt: float, interval: int = 0) -> None:
+        """nsteps fixed-dt steps of the state with checkpoints every
+        `interval` steps (0: ceil(sqrt(nsteps)))."""
+        check(self.L.nest_hip_sim2d_rollout(self.h, nsteps, dt, interval))
+
+    def rollout_adjoint(self, Ubar: int) -> None:
+        """Cotangent of the last rollout's final state -> of its initial state, in place."""
+        check(self.L.nest_hip_sim2d_rollout_adjoint(self.h, C.c_void_p(Ubar)))
+
+    def rollout_info(self):
+        k, m, dt = C.c_int(), C.c_int(), C.c_double()
+        check(self.L.nest_hip_sim2d_rollout_info(self.h, C.byref(k), C.byref(m), C.byref(dt)))
+        return k.value, m.value, dt.value
+
     def close(self) -> None:
         if getattr(self, "h", None):
             self.L.nest_hip_sim2d_destroy(self.h)

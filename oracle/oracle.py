@@ -84,6 +84,8 @@ class Oracle:
         self._mrun = s("or_mesh_run", _i, [_dp, _i, _i, _i, _i, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _dp])
         self._mjvp = s("or_mesh_rk2_jvp", None,
                        [_dp, _dp, _i, _i, _i, _i, _i, _i, _i, _i, _d, _d, _d, _d, _i])
+        self._msjvp = s("or_mesh_stage_jvp", None,
+                        [_dp, _dp, _i, _i, _i, _i, _i, _i, _i, _d, _d, _d, _d, _i])
 
     # ---- scalar / batched numerics ----
     def cons_to_prim(self, U, gamma=1.4):
@@ -175,6 +177,10 @@ class Oracle:
 
     def mesh_rk2_jvp(self, U, dU, m, K, dt, dx, dy, gamma=1.4, lim="minmod"):
         self._mjvp(U, dU, m.nx, m.ny, m.npx, m.npy, m.h, BC[m.bcx], BC[m.bcy], K, dt, dx, dy, gamma, LIM[lim])
+
+    def mesh_stage_jvp(self, U, dU, m, dt, dx, dy, gamma=1.4, lim="minmod"):
+        """One split stage S = Y o fill o X o fill on dual numbers: U <- S(U), dU <- S'(U) dU."""
+        self._msjvp(U, dU, m.nx, m.ny, m.npx, m.npy, m.h, BC[m.bcx], BC[m.bcy], dt, dx, dy, gamma, LIM[lim])
 
 
 class RefHarness:

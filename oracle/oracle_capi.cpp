@@ -162,4 +162,18 @@ void or_mesh_rk2_jvp(double* U, double* dU, int nx, int ny, int npx, int npy, in
     }
 }
 
+// Forward-mode derivative of one split stage S = Y o fill o X o fill.
+void or_mesh_stage_jvp(double* U, double* dU, int nx, int ny, int npx, int npy, int h, int bcx, int bcy,
+                       double dt, double dx, double dy, double g, int lim) {
+    mesh2d m(nx, ny, npx, npy, h, bcx, bcy);
+    std::vector<dual> X(m.total), s(m.total);
+    for (size_t i = 0; i < m.total; ++i) X[i] = dual(U[i], dU[i]);
+    s = X;
+    split_stage_mesh(X.data(), s.data(), m, dt, dx, dy, g, lim);
+    for (size_t i = 0; i < m.total; ++i) {
+        U[i] = X[i].v;
+        dU[i] = X[i].d;
+    }
+}
+
 }  // extern "C"

@@ -22,6 +22,16 @@ def test_core_patch_pipeline_api():
     assert "0 failed checks" in r.stdout
 
 
+def test_adjoint_building_blocks_match_dual_numbers():
+    """Reverse-mode prim / limiter / HLLE+passive face functions of the
+    adjoint kernels, compiled for the host, against dual-number JVPs of the
+    oracle numerics: <w, J v> == <J^T w, v> to 1e-12 over every branch."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpp"), "_bin/test_adjoint_face"], check=True)
+    r = subprocess.run([os.path.join(BIN, "test_adjoint_face")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS" in r.stdout
+
+
 def test_backend_hip_header_compiles_and_links():
     """exec::hip driver builds against include/nest/backend_hip.hpp and links
     libnest_hip.so (no GPU needed to build)."""
