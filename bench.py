@@ -102,6 +102,89 @@ def ensemble_leg(steps: int, mode: str):
             "tiling": {k: v for k, v in cfg.items() if k in ("1", "2")}}
 
 
+def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps: int = 3):
+    """Config 4: 1024^2 periodic smooth state, K fixed-dt RK2 steps, loss =
+    L2 misfit of the final density against the run from a seeded perturbed
+    initial state. Times (median of `reps`): the plain forward rollout (one
+    checkpoint), the forward with ceil(sqrt K) checkpoints, and the whole
+    gradient (checkpointed forward + reverse sweep); cost = gradient / plain
+    forward. Then checks the gradient against central differences along
+    `ndir` seeded smooth directions of the full state at a generic point
+    next to the config-4 state (its rho, u, v, p plus a seeded 1e-3 smooth
+    field each; the config-4 state's exactly uniform u, v, p put every
+    velocity / pressure limiter slope on its kink, where the loss has no
+    two-sided derivative)."""
+    import numpy as np
+    import torch
+
+    from nest_amd import problems as P
+    from nest_amd.adjoint import Rollout, density_misfit
+
+    n = 1024
+    U0, Ut, dt = P.adjoint_setup(n)
+    ro = Rollout.mesh(n, n, 4, 4, nsteps=K, dt=dt, mode=mode)
+    area = 1.0 / (n * n)
+    with torch.no_grad():
+        target = ro(torch.from_numpy(Ut).cuda())[0].clone()
+
+    def loss(U):
+        return density_misfit(ro(U), target, area)
+
+    def timed(fn):
+        out = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            out.append(time.perf_counter() - t0)
+        return sorted(out[1:])[len(out[1:]) // 2]
+
+    def gradient(U0d):
+        U = U0d.clone().requires_grad_(True)
+        loss(U).backward()
+        return U.grad
+
+    U0d = torch.from_numpy(U0).cuda()
+
+    def fwd_plain():
+        ro.interval = K
+        with torch.no_grad():
+            loss(U0d)
+
+    def fwd_ckpt():
+        ro.interval = 0
+        with torch.no_grad():
+            loss(U0d)
+
+    t_fwd = timed(fwd_plain)
+    t_ck = timed(fwd_ckpt)
+    t_grad = timed(lambda: gradient(U0d))
+    ro.interval = 0
+    Ug = torch.from_numpy(P.adjoint_generic_state(n)).cuda()
+    g = gradient(Ug)
+    d_adj, d_fd = [], []
+    with torch.no_grad():
+        for k in range(ndir):
+            v = torch.from_numpy(P.smooth_direction(n, 200 + k)).cuda()
+            d_adj.append(float(torch.sum(g * v)))
+            d_fd.append((float(loss(Ug + eps * v)) - float(loss(Ug - eps * v))) / (2 * eps))
+    a, f = np.array(d_adj), np.array(d_fd)
+    proj_err = float(np.linalg.norm(f - a) / np.linalg.norm(a))
+    worst = float(np.max(np.abs(f - a) / np.abs(a)))
+    ro.close()
+    cells = n * n
+    return {"workload": f"1024^2 periodic smooth state, K={K} fixed-dt RK2 steps, density L2 misfit, "
+                        f"checkpoints every {int(np.ceil(np.sqrt(K)))} steps",
+            "forward_ms": round(t_fwd * 1e3, 2), "forward_ckpt_ms": round(t_ck * 1e3, 2),
+            "gradient_ms": round(t_grad * 1e3, 2), "adjoint_fwd_cost": round(t_grad / t_fwd, 3),
+            "reverse_only_cost": round((t_grad - t_ck) / t_fwd, 3),
+            "forward_mcups": round(cells * 2.0 * K / t_fwd / 1e6, 1),
+            "fd_point": "config-4 state + seeded 1e-3 smooth perturbation of rho, u, v, p",
+            "fd_directions": ndir, "fd_eps": eps, "fd_projected_rel_l2_err": float(f"{proj_err:.3e}"),
+            "fd_max_rel_err_per_direction": float(f"{worst:.3e}")}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic_stage2d_sod4096.json")
     if os.path.exists(path):
@@ -122,6 +205,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--per-patch-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=100, help="config-3 leg; 0 skips it")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="config-4 leg; 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU tier); "
@@ -192,6 +276,11 @@ def main():
         log("bench: ensemble (512 problems)")
         ens = ensemble_leg(args.ensemble_steps, args.mode)
 
+    adj = None
+    if args.adjoint_steps > 0:
+        log("bench: adjoint (config 4)")
+        adj = adjoint_leg(args.mode, K=args.adjoint_steps)
+
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -221,6 +310,7 @@ def main():
         "graph_value": round(graph_value, 1),
         "per_patch_value": pp,
         "ensemble": ens,
+        "adjoint": adj,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (traffic or {}).get("bytes_per_launch")},

@@ -113,6 +113,14 @@ def load(path: str = LIB_PATH):
         raise NestLibraryMissing(
             f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
             "(no CPU fallback exists for the HIP path)")
+    # One HIP runtime per process: the torch wheel bundles its own HIP / HSA
+    # runtime (same soname), which cannot open the GPU once another copy has.
+    # Loading torch first (when installed) makes this library bind to it, so
+    # both can share the device and its streams.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)

@@ -358,7 +358,7 @@ struct nest_sim2d {
         for (double** p : {&adj_U1, &adj_Vb, &adj_Ub1})
             if (!*p) {
                 if (int rc = dev_alloc(p, total)) return rc;
-                NEST_HIP_TRY(hipMemset(*p, 0, total * sizeof(double)));
+                NEST_HIP_TRY(hipMemsetAsync(*p, 0, total * sizeof(double), stream));
             }
         return NEST_OK;
     }

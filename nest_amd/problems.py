@@ -132,3 +132,71 @@ def ensemble_batch(nprob=512, seed=3, mode="fast", limiter="minmod", Sim2D=None)
     sim.upload(U)
     sim.fill_halos()
     return sim, sizes, sim.download()
+
+
+def adjoint_setup(n=1024, seed=4, cfl=0.4, perturb=1e-2, gamma=1.4):
+    """Config-4 setup on n x n periodic cells: the smooth initial state
+    (smooth_periodic), the initial state of the target run (density times
+    1 + perturb*(2 xi - 1), xi ~ U[0,1) per cell from `seed` + 1, energy
+    adjusted to keep p = 1), and the fixed dt = cfl*dx / max(|u| + c) of the
+    initial state (the larger of the x and y speeds; dx = dy = 1/n)."""
+    U0 = smooth_periodic(n, seed=seed, gamma=gamma)
+    rho0 = U0[0]
+    idx = np.arange(n * n, dtype=np.uint64).reshape(n, n)
+    rho_t = rho0 * (1.0 + perturb * (2.0 * uniform01(seed + 1, idx) - 1.0))
+    u = np.full_like(rho0, 0.1)
+    Ut = np.stack([rho_t, rho_t * u, rho_t * u, energy2d(rho_t, u, u, np.ones_like(rho0), gamma)])
+    c = np.sqrt(gamma * 1.0 / np.minimum(rho0, rho_t))
+    dt = cfl * (1.0 / n) / float(np.max(0.1 + c))
+    return U0, Ut, dt
+
+
+def density_direction(n, seed, kmax=4, u=0.1, v=0.1):
+    """A direction in conserved-state space that changes the density only
+    (velocity and pressure held): drho = random smooth periodic field (the
+    Fourier modes |kx|, |ky| <= kmax with seeded amplitudes and phases,
+    scaled to max |drho| = 1); returns (drho, u drho, v drho,
+    0.5 (u^2 + v^2) drho)."""
+    r = uniform01(seed, np.arange(4 * (2 * kmax + 1) ** 2, dtype=np.uint64)).reshape(-1, 4)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    d = np.zeros((n, n))
+    i = 0
+    for kx in range(-kmax, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            a, ph = 2.0 * r[i, 0] - 1.0, 2.0 * np.pi * r[i, 1]
+            d += a * np.cos(2.0 * np.pi * (kx * X + ky * Y) + ph)
+            i += 1
+    d /= np.max(np.abs(d))
+    return np.stack([d, u * d, v * d, 0.5 * (u * u + v * v) * d])
+
+
+def smooth_field(n, seed, kmax=4):
+    """Seeded smooth periodic field on n x n cells (Fourier modes |kx|, |ky|
+    <= kmax, random amplitudes and phases), scaled to max |f| = 1."""
+    return density_direction(n, seed, kmax=kmax, u=0.0, v=0.0)[0]
+
+
+def smooth_direction(n, seed, kmax=4):
+    """Direction in conserved-state space: an independent seeded smooth
+    field per conserved variable (seeds 4*seed .. 4*seed + 3)."""
+    return np.stack([smooth_field(n, 4 * seed + k, kmax) for k in range(4)])
+
+
+def adjoint_generic_state(n=1024, seed=4, amp=1e-3, gamma=1.4):
+    """The config-4 state made generic: rho, u, v, p of adjoint_setup's
+    initial state each plus amp times a seeded smooth field. (The config-4
+    state itself has exactly uniform u, v, p, so every velocity / pressure
+    limiter slope sits on its kink and the loss has no two-sided derivative
+    there along directions that move u, v or p.)"""
+    U0, _, _ = adjoint_setup(n, seed=seed, gamma=gamma)
+    r = U0[0]
+    u = U0[1] / r
+    v = U0[2] / r
+    p = (gamma - 1.0) * (U0[3] - 0.5 * r * (u * u + v * v))
+    base = 7919 + 10 * seed
+    r = r + amp * smooth_field(n, base)
+    u = u + amp * smooth_field(n, base + 1)
+    v = v + amp * smooth_field(n, base + 2)
+    p = p + amp * smooth_field(n, base + 3)
+    return np.stack([r, r * u, r * v, energy2d(r, u, v, p, gamma)])

@@ -1,0 +1,183 @@
+"""GPU: the torch.autograd.Function over the C ABI (nest_amd.adjoint).
+
+Config-4 loss: L2 misfit of the final density after K fixed-dt RK2 steps of
+the smooth periodic state. The adjoint gradient is checked against
+* forward-mode dual numbers through the oracle (the same branch decisions as
+  the primal), at the config-4 state itself, along seeded random directions
+  of the full conserved state: relative error <= 1e-10 (exact numerics mode,
+  whose primal is bitwise the oracle's);
+* central finite differences along seeded random smooth directions of the
+  full conserved state, at a generic point next to the config-4 state (its
+  rho, u, v, p plus a seeded 1e-3 smooth field each): the vector of
+  directional derivatives (the gradient projected on the directions) within
+  1e-6 relative L2.
+  The config-4 state itself is not a point where the loss is differentiable:
+  its u, v, p are exactly uniform, so every velocity and pressure limiter
+  slope has both one-sided differences at rounding level and sits on its
+  kink. Central differences there converge to branch averages that no
+  gradient reproduces; the agreement measured there (about 1e-4) is printed
+  by the config-4 test, not asserted. Minmod kinks are also dense at a
+  generic point (the central-difference error falls about as sqrt(eps)), so
+  the difference step is 1e-9 on directions scaled to max |v| = 1.
+Batched adjoints (many equal-size problems in one handle) must equal the
+single-problem gradients bitwise, and a second forward before backward is
+refused (its checkpoints are gone).
+"""
+import numpy as np
+import pytest
+
+from nest_amd import problems as P
+from oracle.oracle import MeshSpec
+from tests.gpu_util import print_case
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FD_TOL = 1e-6
+FD_EPS = 1e-9
+DUAL_TOL = 1e-10
+
+
+def _setup(n, K, mode, npx=1, npy=1, interval=0, U0=None):
+    from nest_amd.adjoint import Rollout, density_misfit
+    U0c, Ut, dt = P.adjoint_setup(n)
+    if U0 is None:
+        U0 = U0c
+    ro = Rollout.mesh(n, n, npx, npy, nsteps=K, dt=dt, bc="periodic", mode=mode, interval=interval)
+    U0d = torch.from_numpy(U0).cuda()
+    with torch.no_grad():
+        target = ro(torch.from_numpy(Ut).cuda())[0].clone()
+    area = 1.0 / (n * n)
+
+    def loss(U):
+        return density_misfit(ro(U), target, area)
+
+    U = U0d.clone().requires_grad_(True)
+    loss(U).backward()
+    return ro, U0, U0d, loss, U.grad, target, dt
+
+
+def _fd_check(loss, U0d, grad, dirs, eps_list):
+    """Central differences along each direction at every eps of eps_list.
+    Returns, for the first eps, the relative L2 error of the vector of
+    directional derivatives (the gradient projected on the directions):
+    per-direction relative errors are ill-conditioned for a direction whose
+    derivative happens to be small (they are printed)."""
+    d_adj, d_fd = [], []
+    with torch.no_grad():
+        for vn in dirs:
+            v = torch.from_numpy(vn).cuda()
+            d = float(torch.sum(grad * v))
+            fds = [(float(loss(U0d + eps * v)) - float(loss(U0d - eps * v))) / (2 * eps) for eps in eps_list]
+            print(f"  FD  dL/dv {d:+.9e}  rel err {['%.0e: %.1e' % (e, abs(f - d) / abs(d)) for e, f in zip(eps_list, fds)]}")
+            d_adj.append(d)
+            d_fd.append(fds[0])
+    a, f = np.array(d_adj), np.array(d_fd)
+    err = float(np.linalg.norm(f - a) / np.linalg.norm(a))
+    print(f"  FD projected-gradient relative L2 error ({len(dirs)} directions, eps {eps_list[0]:.0e}): {err:.2e}")
+    return err
+
+
+def _dual_check(oracle, U0, grad, target, n, npx, npy, K, dt, ndir, seed):
+    """<grad, v> against dL = sum area (rho_K - rho_t) drho_K with (U_K, dU_K)
+    from the oracle's dual-number rollout; v random in the full state."""
+    m = MeshSpec(n, n, npx, npy, bcx="periodic", bcy="periodic")
+    g = grad.detach().cpu().numpy()
+    tgt = target.detach().cpu().numpy()
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for _ in range(ndir):
+        v = rng.standard_normal(U0.shape)
+        Up, dUp = m.to_patches(U0), m.to_patches(v)
+        oracle.mesh_rk2_jvp(Up, dUp, m, K, dt, 1.0 / n, 1.0 / n)
+        UK, dUK = m.to_global(Up), m.to_global(dUp)
+        d_dual = float(np.sum((UK[0] - tgt) * dUK[0])) / (n * n)
+        d_adj = float(np.sum(g * v))
+        err = abs(d_dual - d_adj) / abs(d_dual)
+        print(f"  dual dL/dv {d_dual:+.12e} adjoint {d_adj:+.12e} rel {err:.2e}")
+        worst = max(worst, err)
+    return worst
+
+
+def test_torch_gradient_small(oracle):
+    """96^2, 2x3 patches, K = 12, checkpoints every 5 steps: dual numbers at
+    the config-4 state, central differences at the generic point."""
+    print_case("torch rollout gradient 96^2 K=12")
+    ro, U0, U0d, loss, grad, target, dt = _setup(96, 12, "exact", npx=2, npy=3, interval=5)
+    e_dual = _dual_check(oracle, U0, grad, target, 96, 2, 3, 12, dt, 4, 5)
+    ro.close()
+    ro, U0, U0d, loss, grad, target, dt = _setup(96, 12, "exact", npx=2, npy=3, interval=5,
+                                                 U0=P.adjoint_generic_state(96))
+    e_fd = _fd_check(loss, U0d, grad, [P.smooth_direction(96, 50 + k) for k in range(4)], [FD_EPS, 1e-8, 1e-7])
+    ro.close()
+    assert e_fd <= FD_TOL and e_dual <= DUAL_TOL, (e_fd, e_dual)
+
+
+def test_config4_gradient_vs_central_fd():
+    """Config 4 size and rollout: 1024^2 periodic, K = 200 fixed-dt steps,
+    sqrt(K) = 15-step checkpoint intervals, 16 seeded smooth directions of
+    the full state at the generic point (fast numerics). The agreement at
+    the config-4 state itself along density-only directions is printed."""
+    print_case("config 4 gradient vs FD, 1024^2 K=200")
+    ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "fast", npx=4, npy=4,
+                                                 U0=P.adjoint_generic_state(1024))
+    k, m, _ = ro.sim.rollout_info()
+    assert k == 200 and m == 15
+    worst = _fd_check(loss, U0d, grad, [P.smooth_direction(1024, 200 + k) for k in range(16)], [FD_EPS, 1e-8])
+    ro.close()
+    print("  at the config-4 state itself (density-only directions, not asserted):")
+    ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "fast", npx=4, npy=4)
+    _fd_check(loss, U0d, grad, [P.density_direction(1024, 300 + k) for k in range(2)], [1e-6, 1e-8])
+    ro.close()
+    assert worst <= FD_TOL, worst
+
+
+def test_config4_gradient_vs_dual_numbers(oracle):
+    """Config 4 at full size, at the config-4 state, against the oracle's
+    dual-number rollout along a random direction of the full conserved
+    state (exact numerics)."""
+    print_case("config 4 gradient vs dual numbers, 1024^2 K=200")
+    ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "exact", npx=4, npy=4)
+    worst = _dual_check(oracle, U0, grad, target, 1024, 4, 4, 200, dt, 1, 7)
+    assert worst <= DUAL_TOL, worst
+    ro.close()
+
+
+def test_batched_adjoints_equal_single():
+    """64 problems of 256^2 in one batch handle (config 5's batched adjoints,
+    config 4's setup per problem with its own seed) vs the same problems one
+    at a time: gradients bitwise equal."""
+    from nest_amd.adjoint import Rollout, density_misfit
+    print_case("batched adjoints 64 x 256^2")
+    n, B, K = 256, 64, 20
+    U0s, Uts, dts = zip(*[P.adjoint_setup(n, seed=10 + 3 * b) for b in range(B)])
+    dt = min(dts)
+    area = 1.0 / (n * n)
+    rb = Rollout.batch(B, n, n, nsteps=K, dt=dt, mode="fast")
+    U0 = torch.from_numpy(np.stack(U0s)).cuda()
+    with torch.no_grad():
+        tgt = rb(torch.from_numpy(np.stack(Uts)).cuda())[:, 0].clone()
+    U = U0.clone().requires_grad_(True)
+    density_misfit(rb(U), tgt, area).backward()
+    gb = U.grad
+    for b in (0, 17, 63):
+        rs = Rollout.mesh(n, n, nsteps=K, dt=dt, mode="fast")
+        u = U0[b].clone().requires_grad_(True)
+        density_misfit(rs(u), tgt[b], area).backward()
+        assert torch.equal(u.grad, gb[b]), f"problem {b}"
+        rs.close()
+    rb.close()
+
+
+def test_second_forward_invalidates_first_graph():
+    from nest_amd.adjoint import Rollout
+    print_case("rollout token guard")
+    U0, _, dt = P.adjoint_setup(64)
+    ro = Rollout.mesh(64, 64, nsteps=3, dt=dt, mode="fast")
+    a = torch.from_numpy(U0).cuda().requires_grad_(True)
+    y1 = ro(a)
+    y2 = ro(a)
+    y2.sum().backward(retain_graph=True)
+    with pytest.raises(RuntimeError):
+        y1.sum().backward()
+    ro.close()
