@@ -123,6 +123,9 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
     n = 1024
     U0, Ut, dt = P.adjoint_setup(n)
     ro = Rollout.mesh(n, n, 4, 4, nsteps=K, dt=dt, mode=mode)
+    sizes = [(n, n)]
+    ro.sim.upload(ro.sim.scatter(U0))
+    fwd_cfg, adj_cfg = ro.tune(sizes, log=log)
     area = 1.0 / (n * n)
     with torch.no_grad():
         target = ro(torch.from_numpy(Ut).cuda())[0].clone()
@@ -180,6 +183,8 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
             "gradient_ms": round(t_grad * 1e3, 2), "adjoint_fwd_cost": round(t_grad / t_fwd, 3),
             "reverse_only_cost": round((t_grad - t_ck) / t_fwd, 3),
             "forward_mcups": round(cells * 2.0 * K / t_fwd / 1e6, 1),
+            "tiling": {"forward": {k: v for k, v in fwd_cfg.items() if k in ("1", "2")},
+                       "adjoint": {k: v for k, v in adj_cfg.items() if k != "ms_per_vjp"}},
             "fd_point": "config-4 state + seeded 1e-3 smooth perturbation of rho, u, v, p",
             "fd_directions": ndir, "fd_eps": eps, "fd_projected_rel_l2_err": float(f"{proj_err:.3e}"),
             "fd_max_rel_err_per_direction": float(f"{worst:.3e}")}

@@ -41,15 +41,31 @@ class Rollout:
 
     @classmethod
     def mesh(cls, nx, ny, npx=1, npy=1, nsteps=1, dt=1e-3, bc="periodic", limiter="minmod", mode="fast",
-             interval=0, gamma=1.4):
+             interval=0, gamma=1.4, tune=False):
         sim = Sim2D.mesh(nx, ny, npx, npy, gamma=gamma, bcx=bc, bcy=bc, limiter=limiter, mode=mode)
-        return cls(sim, (4, nx, ny), nsteps, dt, interval)
+        ro = cls(sim, (4, nx, ny), nsteps, dt, interval)
+        if tune:
+            ro.tune([(nx, ny)])
+        return ro
 
     @classmethod
     def batch(cls, nprob, nx, ny, nsteps=1, dt=1e-3, bc="periodic", limiter="minmod", mode="fast", interval=0,
-              gamma=1.4):
+              gamma=1.4, tune=False):
         sim = Sim2D.batch([(nx, ny)] * nprob, gamma=gamma, bc=bc, limiter=limiter, mode=mode)
-        return cls(sim, (nprob, 4, nx, ny), nsteps, dt, interval)
+        ro = cls(sim, (nprob, 4, nx, ny), nsteps, dt, interval)
+        if tune:
+            ro.tune([(nx, ny)] * nprob)
+        return ro
+
+    def tune(self, sizes, log=None):
+        """Autotune the forward stage and adjoint kernels' tiling on this
+        handle (results cached per device and problem-size signature). The
+        handle's state is overwritten (every forward call reloads it)."""
+        from . import autotune
+        self.sim.start(cfl=0.4)
+        fwd = autotune.tune(self.sim, sizes, log=log)
+        adj = autotune.tune_adjoint(self.sim, sizes, self.dt, log=log)
+        return fwd, adj
 
     def __call__(self, U0: torch.Tensor) -> torch.Tensor:
         if U0.dtype != torch.float64 or not U0.is_cuda or tuple(U0.shape) != self.shape:

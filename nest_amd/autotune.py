@@ -10,15 +10,20 @@ few device-dt steps with HIP events on the handle's stream (the same timing
 path as bench.py). Results are cached in a JSON file keyed by device name
 and the problem set's size-class histogram, so a later run on the same kind
 of input starts from the tuned point.
+
+tune_adjoint() does the same for the two adjoint kernels (rows per work item
+and waves per block of each), timing whole RK2-step vector-Jacobian products.
 """
 from __future__ import annotations
 
 import json
 import os
+import time
 
 import numpy as np
 
-ROWS = (16, 24, 32, 48, 64, 96, 128)
+ROWS = (8, 12, 16, 24, 32, 48, 64, 96, 128)
+ADJ_ROWS = (8, 12, 16, 24, 32, 48)
 WAVES = (1, 2, 4)
 
 
@@ -54,18 +59,78 @@ def apply(sim, classes, cfg) -> None:
         sim.set_problem_rows(st, [c["rows"][k] for k in classes])
 
 
+def _load(path):
+    if os.path.exists(path):
+        try:
+            return json.load(open(path))
+        except (OSError, ValueError):
+            return {}
+    return {}
+
+
+def _store(path, table):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        json.dump(table, open(path, "w"), indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 3, cache: str | None = None, log=None):
+    """Tune the adjoint kernels' tiling of `sim` in place: coordinate descent
+    over rows and waves per block of the Y and X kernels, each candidate
+    timed over `reps` RK2-step VJPs (the handle's own state as U0 and its
+    stage scratch as the cotangent, so no extra memory). Returns
+    {"rows_y", "waves_y", "rows_x", "waves_x"}."""
+    classes = size_classes(sizes)
+    key = "adjoint|" + signature(sizes, classes, arch)
+    path = _cache_path(cache)
+    table = _load(path)
+    if key in table:
+        c = table[key]
+        sim.set_adjoint_tiling(c["rows_y"], c["waves_y"], c["rows_x"], c["waves_x"])
+        return c
+    U, U1 = sim.device_ptrs()
+
+    def run(c):
+        sim.set_adjoint_tiling(c["rows_y"], c["waves_y"], c["rows_x"], c["waves_x"])
+        sim.rk2_vjp(U, U1, dt)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sim.rk2_vjp(U, U1, dt)
+        sim.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    cfg = {"rows_y": 32, "waves_y": 4, "rows_x": 32, "waves_x": 4}
+    best = run(cfg)
+    ntrials = 0
+    for field, values in (("rows_y", ADJ_ROWS), ("rows_x", ADJ_ROWS), ("waves_y", WAVES), ("waves_x", WAVES)):
+        for v in values:
+            if v == cfg[field]:
+                continue
+            trial = dict(cfg)
+            trial[field] = v
+            t = run(trial)
+            ntrials += 1
+            if t < best:
+                best, cfg = t, trial
+    sim.set_adjoint_tiling(cfg["rows_y"], cfg["waves_y"], cfg["rows_x"], cfg["waves_x"])
+    cfg["ms_per_vjp"] = round(best * 1e3, 4)
+    if log:
+        log(f"autotune {key}: {cfg} ({ntrials} trials)")
+    table[key] = cfg
+    _store(path, table)
+    return cfg
+
+
 def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = None, log=None):
     """Tune `sim` (already started: start() called) in place and return the
     configuration; `sizes` = [(nx, ny)] per problem."""
     classes = size_classes(sizes)
     key = signature(sizes, classes, arch)
     path = _cache_path(cache)
-    table = {}
-    if os.path.exists(path):
-        try:
-            table = json.load(open(path))
-        except (OSError, ValueError):
-            table = {}
+    table = _load(path)
     if key in table:
         apply(sim, classes, table[key])
         return table[key]
@@ -103,9 +168,5 @@ def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = N
     if log:
         log(f"autotune {key}: {cfg} ({len(trials)} trials)")
     table[key] = cfg
-    try:
-        os.makedirs(os.path.dirname(path), exist_ok=True)
-        json.dump(table, open(path, "w"), indent=1, sort_keys=True)
-    except OSError:
-        pass
+    _store(path, table)
     return cfg

@@ -78,7 +78,11 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
         const double dtdy = dt / ps->dy;
 
         const double* Ubuf = STAGE == 1 ? a.U : a.U1;
-        double* Obuf = STAGE == 1 ? a.U1 : a.U;
+        // stage 2 writes 0.5 (U^n + S(U1)) to a.Uo when given (U^n is then
+        // left intact), else in place of U^n; U^n is read at the output cell
+        // through the wave-uniform offset u0_off from the output pointer
+        double* Obuf = STAGE == 1 ? a.U1 : (a.Uo ? a.Uo : a.U);
+        const ptrdiff_t u0_off = a.U - Obuf;
         // Per lane: where this lane's column lives for interior rows (base,
         // row length, variable stride) and where its output goes.
         const double* base;
@@ -176,10 +180,11 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             double U0[4];  // U^n at the output cell (stage 2 combine), issued early
             if constexpr (STAGE == 2) {
                 if (store) {
-                    U0[0] = po[0];
-                    U0[1] = po[vs];
-                    U0[2] = po[2 * (size_t)vs];
-                    U0[3] = po[3 * (size_t)vs];
+                    const double* pu = po + u0_off;
+                    U0[0] = pu[0];
+                    U0[1] = pu[vs];
+                    U0[2] = pu[2 * (size_t)vs];
+                    U0[3] = pu[3 * (size_t)vs];
                 }
             }
             prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);

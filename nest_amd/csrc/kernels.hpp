@@ -25,7 +25,7 @@
 
 // arguments of the 2D stage / speed kernels
 struct stage_args {
-    double* U;   // U^n, updated in place to U^{n+1} by stage 2
+    double* U;   // U^n, updated in place to U^{n+1} by stage 2 unless Uo is set
     double* U1;  // first-stage state
     const nest_patch_desc* patches;
     const nest_item* items;
@@ -39,6 +39,7 @@ struct stage_args {
     int gather;
     const nest_prob_geom* geoms;
     const int* itab;
+    double* Uo;  // stage 2 output (null: in place of U)
 };
 
 // arguments of the two adjoint kernels of one split stage S = Y o X (the

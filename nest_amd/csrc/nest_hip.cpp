@@ -160,6 +160,7 @@ struct nest_sim2d {
     long long* goff_d = nullptr;
     std::vector<double*> ckpt;  // rollout checkpoints (every roll_m steps)
     std::vector<double*> seg;   // recomputed states of one checkpoint interval
+    std::vector<double*> seg1;  // their stage-1 states
     int roll_K = 0, roll_m = 0;
     double roll_dt = 0.0;
 
@@ -178,7 +179,7 @@ struct nest_sim2d {
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
                         (void*)goff_d})
             if (p) (void)hipFree(p);
-        for (auto* v : {&ckpt, &seg}) {
+        for (auto* v : {&ckpt, &seg, &seg1}) {
             for (double* p : *v) (void)hipFree(p);
             v->clear();
         }
@@ -346,12 +347,14 @@ struct nest_sim2d {
     // ---- adjoint and fixed-dt rollouts ----
     // forward stage st on an arbitrary state buffer X (stage 1: X -> X1;
     // stage 2: X <- 0.5 (X + S(X1)) in place), host dt
-    void stage_on(int st, double* X, double* X1, double dt) {
+    // (stage 2 with Xo: Xo <- 0.5 (X + S(X1)), X left intact)
+    void stage_on(int st, double* X, double* X1, double dt, double* Xo = nullptr) {
         const int n = (int)gitems_h[st - 1].size();
         const int w = gwpb[st - 1];
         stage_args a = args(dt, 0, n, st);
         a.U = X;
         a.U1 = X1;
+        a.Uo = Xo;
         L().stage2d(st, lim, a, grid_for(n, w), 64 * w, stream);
     }
     int ensure_adjoint_scratch() {
@@ -399,7 +402,11 @@ struct nest_sim2d {
     // Ub1 = S'(U1)^T (0.5 Ub) and Ub <- 0.5 Ub + S'(U0)^T Ub1.
     void vjp_step(const double* U0, double* Ub, double dt) {
         stage_on(1, const_cast<double*>(U0), adj_U1, dt);
-        adj_stage(adj_U1, Ub, 0.5, nullptr, 0.0, adj_Ub1, dt);
+        vjp_step_u1(U0, adj_U1, Ub, dt);
+    }
+    // the same with the step's stage-1 state U1 = S(U0) already at hand
+    void vjp_step_u1(const double* U0, const double* U1s, double* Ub, double dt) {
+        adj_stage(U1s, Ub, 0.5, nullptr, 0.0, adj_Ub1, dt);
         adj_stage(U0, adj_Ub1, 1.0, Ub, 0.5, Ub, dt);
     }
 };
@@ -879,18 +886,18 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     const int K = s->roll_K, m = s->roll_m;
     const double dt = s->roll_dt;
     if (int rc = s->grow(s->seg, (size_t)std::max(0, m - 1))) return rc;
+    if (int rc = s->grow(s->seg1, (size_t)m)) return rc;
     const int nck = (K + m - 1) / m;
     for (int c = nck - 1; c >= 0; --c) {
         const int len = std::min(m, K - c * m);
-        // states of steps c*m .. c*m + len - 1: the checkpoint, then recomputed
+        // states of steps c*m .. c*m + len - 1 (the checkpoint, then
+        // recomputed) and the stage-1 state of each, kept for the reverse
         auto state = [&](int i) { return i == 0 ? s->ckpt[c] : s->seg[i - 1]; };
-        for (int i = 1; i < len; ++i) {
-            NEST_HIP_TRY(hipMemcpyAsync(state(i), state(i - 1), s->total * sizeof(double),
-                                        hipMemcpyDeviceToDevice, s->stream));
-            s->stage_on(1, state(i), s->U1, dt);
-            s->stage_on(2, state(i), s->U1, dt);
+        for (int i = 0; i < len; ++i) {
+            s->stage_on(1, state(i), s->seg1[i], dt);
+            if (i + 1 < len) s->stage_on(2, state(i), s->seg1[i], dt, state(i + 1));
         }
-        for (int i = len - 1; i >= 0; --i) s->vjp_step(state(i), Ubar, dt);
+        for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(i), s->seg1[i], Ubar, dt);
     }
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
