@@ -68,6 +68,28 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
+    // Per wave: a ring of the last 4 loaded rows (4 variables x 64 lanes each)
+    // in LDS. A row enters the X window 3 iterations before it is the centre
+    // row whose conserved state the update needs; keeping it here instead of
+    // re-reading it from memory halves the stage's HBM reads (the L2 does not
+    // hold rows that long at full occupancy). Each lane touches only its own
+    // column, so no barrier is needed.
+    __shared__ double ring_all[4 * 4 * 4 * 64];
+    double* const ring = ring_all + (threadIdx.x >> 6) * (4 * 4 * 64) + lane;
+    auto ring_put = [&](int e, const double* u) {
+        double* q = ring + (e & 3) * (4 * 64);
+        q[0] = u[0];
+        q[64] = u[1];
+        q[128] = u[2];
+        q[192] = u[3];
+    };
+    auto ring_get = [&](int e, double* u) {
+        const double* q = ring + (e & 3) * (4 * 64);
+        u[0] = q[0];
+        u[1] = q[64];
+        u[2] = q[128];
+        u[3] = q[192];
+    };
     for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
@@ -149,8 +171,10 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             load_row(w.r0 + 1, A);
             prim4 Wm1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
             load_row(w.r0 + 2, A);
+            ring_put(w.r0 + 2, A);
             prim4 W0 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
             load_row(w.r0 + 3, A);
+            ring_put(w.r0 + 3, A);
             W1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
             prim4 d0 = sub4(W0, Wm1);
             dW1 = sub4(W1, W0);
@@ -165,17 +189,10 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
         load_row(w.r0 + 4, Un);
         for (int i = w.r0; i < w.r1; ++i) {
             // prefetch the row the next iteration needs (clamped in bounds on the
-            // last), and re-read this row's conserved state (cache-resident: it
-            // was loaded two iterations ago) rather than carrying it
+            // last); this row's conserved state comes from the LDS ring
             double Unx[4], Uc[4];
             load_row(min(i + 5, pd.nx + 3), Unx);
-            {
-                const double* pc = base + (size_t)(i + 2) * snye;
-                Uc[0] = ld(pc);
-                Uc[1] = ld(pc + svs);
-                Uc[2] = ld(pc + 2 * (size_t)svs);
-                Uc[3] = ld(pc + 3 * (size_t)svs);
-            }
+            ring_get(i + 2, Uc);
             double* po = out + (size_t)(i + 2) * snye;
             double U0[4];  // U^n at the output cell (stage 2 combine), issued early
             if constexpr (STAGE == 2) {
@@ -187,6 +204,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                     U0[3] = pu[3 * (size_t)vs];
                 }
             }
+            ring_put(i + 4, Un);
             prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
             prim4 dW2 = sub4(W2, W1);
             prim4 s1 = slope4<LIM>(dW1, dW2);
