@@ -35,6 +35,13 @@ __device__ __forceinline__ prim4 shup4(const prim4& a, int d) {
 __device__ __forceinline__ prim4 shdn4(const prim4& a, int d) {
     return {__shfl_down(a.r, d), __shfl_down(a.u, d), __shfl_down(a.p, d), __shfl_down(a.q, d)};
 }
+// delta 1 by DPP wave shifts (same lane semantics as the shuffles)
+__device__ __forceinline__ prim4 up1(const prim4& a) {
+    return {lane_up(a.r), lane_up(a.u), lane_up(a.p), lane_up(a.q)};
+}
+__device__ __forceinline__ prim4 dn1(const prim4& a) {
+    return {lane_down(a.r), lane_down(a.u), lane_down(a.p), lane_down(a.q)};
+}
 
 // ---------------------------------------------------------------------------
 // Y kernel
@@ -139,18 +146,18 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) A[k] = real ? a.g * A[k] : 0.0;
             const prim4 Y = prim2d(v0, v1, v2, v3, false, a.eos);
-            const prim4 dl = sub4(Y, shup4(Y, 1));
-            const prim4 dr = shdn4(dl, 1);
+            const prim4 dl = sub4(Y, up1(Y));
+            const prim4 dr = dn1(dl);
             const prim4 sy = slope4<LIM>(dl, dr);
             const coef4 cf = limiter_coef4<LIM>(dl, dr);
             const prim4 Lf = face_plus(Y, sy);
-            const prim4 Rf = shdn4(face_minus(Y, sy), 1);
+            const prim4 Rf = dn1(face_minus(Y, sy));
             // face between this lane and the next: flux cotangent from the
             // output cotangents of the two cells (y_sweep_t update order:
             // rho <- G0, m_u <- G3, m_v <- G1, E <- G2)
             double An[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) An[k] = __shfl_down(A[k], 1);
+            for (int k = 0; k < 4; ++k) An[k] = lane_down(A[k]);
             double Fb[4];
             Fb[0] = dtdy * (An[0] - A[0]);
             Fb[1] = dtdy * (An[2] - A[2]);
@@ -158,19 +165,19 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
             Fb[3] = dtdy * (An[1] - A[1]);
             prim4 Lb = zero4(), Rb = zero4();
             if (lane >= 1 && lane <= 61) face_flux_rev(Lf, Rf, a.eos, Fb, Lb, Rb);
-            const prim4 Rbm = shup4(Rb, 1);  // right-state cotangent of the face below this cell
+            const prim4 Rbm = up1(Rb);  // right-state cotangent of the face below this cell
             const prim4 sb = scale4(0.5, sub4(Lb, Rbm));
             const prim4 dlb = mul4(cf.l, sb);
             const prim4 drb = mul4(cf.r, sb);
             prim4 Wb = add4(add4(Lb, Rbm), sub4(dlb, drb));
-            Wb = sub4(Wb, shdn4(dlb, 1));
-            Wb = add4(Wb, shup4(drb, 1));
+            Wb = sub4(Wb, dn1(dlb));
+            Wb = add4(Wb, up1(drb));
             if (edge) {
                 // outflow: ghost positions -1, -2 are copies of column 0 and
                 // NY, NY+1 of column NY-1 (same primitives, so the
                 // cotangents add before the primitive reverse)
-                const prim4 u1 = shup4(Wb, 1), u2 = shup4(Wb, 2);
-                const prim4 d1 = shdn4(Wb, 1), d2 = shdn4(Wb, 2);
+                const prim4 u1 = up1(Wb), u2 = shup4(Wb, 2);
+                const prim4 d1 = dn1(Wb), d2 = shdn4(Wb, 2);
                 if (gcol == 0) Wb = add4(Wb, add4(u1, u2));
                 if (gcol == G.NY - 1) Wb = add4(Wb, add4(d1, d2));
             }

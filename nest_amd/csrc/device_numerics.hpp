@@ -217,6 +217,25 @@ __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const 
     }
 }
 
+#if defined(__HIPCC__)
+// Neighbour-lane values by DPP wave shifts (a VALU move, no LDS round trip):
+// lane_up(x) on lane l is x of lane l-1 and lane_down(x) is x of lane l+1;
+// lanes 0 / 63 keep their own value (the semantics of __shfl_up / __shfl_down
+// with delta 1).
+__device__ __forceinline__ double lane_up(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_down(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+#endif
+
 // slope of all 4 quantities from the left and right differences
 template <int LIM>
 __device__ __forceinline__ prim4 slope4(const prim4& dl, const prim4& dr) {

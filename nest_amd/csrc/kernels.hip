@@ -36,10 +36,10 @@ namespace NEST_NS {
 __device__ __forceinline__ double ld(const double* p) { return *p; }
 
 __device__ __forceinline__ prim4 shfl_up4(const prim4& a) {
-    return {__shfl_up(a.r, 1), __shfl_up(a.u, 1), __shfl_up(a.p, 1), __shfl_up(a.q, 1)};
+    return {lane_up(a.r), lane_up(a.u), lane_up(a.p), lane_up(a.q)};
 }
 __device__ __forceinline__ prim4 shfl_down4(const prim4& a) {
-    return {__shfl_down(a.r, 1), __shfl_down(a.u, 1), __shfl_down(a.p, 1), __shfl_down(a.q, 1)};
+    return {lane_down(a.r), lane_down(a.u), lane_down(a.p), lane_down(a.q)};
 }
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             face_flux<true>(face_plus(Y, sy), Rf, a.eos, G);
             double Gm[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) Gm[k] = __shfl_up(G[k], 1);
+            for (int k = 0; k < 4; ++k) Gm[k] = lane_up(G[k]);
             double o0 = ux0 - dtdy * (G[0] - Gm[0]);
             double o1 = ux1 - dtdy * (G[3] - Gm[3]);
             double o2 = ux2 - dtdy * (G[1] - Gm[1]);

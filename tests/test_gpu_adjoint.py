@@ -212,6 +212,7 @@ def test_pack_unpack_roundtrip():
     sim.synchronize()
     assert np.array_equal(host(g).reshape(4, nx, ny), G)
     back = torch.zeros_like(Ud)
+    torch.cuda.synchronize()  # the zero fill runs on torch's stream, unpack on the handle's
     sim.unpack(g.data_ptr(), back.data_ptr())
     sim.synchronize()
     assert np.array_equal(sim.gather(host(back)), G)
