@@ -48,13 +48,18 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 #endif
 
 #if NEST_FAST
-// 1/x: hardware estimate + two Newton-Raphson steps (error ~1 ulp)
+#ifndef NEST_NR_STEPS
+#define NEST_NR_STEPS 2
+#endif
+// 1/x: hardware estimate (relative error 2^-24.3) + Newton-Raphson steps
+// (two: ~1 ulp; one: ~2^-48)
 __device__ __forceinline__ double rcp(double x) {
     double y = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
+#pragma unroll
+    for (int k = 0; k < NEST_NR_STEPS; ++k) {
+        const double e = __builtin_fma(-x, y, 1.0);
+        y = __builtin_fma(y, e, y);
+    }
     return y;
 }
 __device__ __forceinline__ double ddiv(double a, double b) { return a * rcp(b); }
@@ -140,8 +145,8 @@ __device__ __forceinline__ prim4 prim1d(double rho, double m, double E, const eo
 __device__ __forceinline__ double rsq(double x) {
     double y = __builtin_amdgcn_rsq(x);
     const double h = 0.5 * x;
-    y = y * __builtin_fma(-h, y * y, 1.5);
-    y = y * __builtin_fma(-h, y * y, 1.5);
+#pragma unroll
+    for (int k = 0; k < NEST_NR_STEPS; ++k) y = y * __builtin_fma(-h, y * y, 1.5);
     return y;
 }
 #endif

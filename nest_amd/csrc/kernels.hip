@@ -185,14 +185,37 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
         }
 
         unsigned long long smx = 0, smy = 0;
-        double Un[4];
-        load_row(w.r0 + 4, Un);
-        for (int i = w.r0; i < w.r1; ++i) {
-            // prefetch the row the next iteration needs (clamped in bounds on the
-            // last); this row's conserved state comes from the LDS ring
-            double Unx[4], Uc[4];
-            load_row(min(i + 5, pd.nx + 3), Unx);
-            ring_get(i + 2, Uc);
+        double UA[4], UB[4];  // rows e = r0+4 (and r0+5, stage 1) for the first X steps
+        load_row(w.r0 + 4, UA);
+        if constexpr (STAGE == 1) load_row(min(w.r0 + 5, pd.nx + 3), UB);
+        // X step of row j: consumes the window and the row loaded for it (Ub,
+        // extended row j+4), refills Ub with the row two steps ahead (two row
+        // buffers alternate, so each load has two steps to arrive), writes
+        // the X-swept state of row j to ux (x_sweep_t update,
+        // euler2d.hpp:144-148)
+        auto xstep = [&](int j, double* ux, double* Ub, int ahead) {
+            double Uc[4];
+            ring_get(j + 2, Uc);
+            ring_put(j + 4, Ub);
+            prim4 W2 = prim2d(Ub[0], Ub[1], Ub[2], Ub[3], true, a.eos);
+            load_row(min(j + 4 + ahead, pd.nx + 3), Ub);
+            prim4 dW2 = sub4(W2, W1);
+            prim4 s1 = slope4<LIM>(dW1, dW2);
+            double Fp[4];
+            face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
+            FL = face_plus(W1, s1);
+            ux[0] = Uc[0] - dtdx * (Fp[0] - Fm[0]);
+            ux[1] = Uc[1] - dtdx * (Fp[1] - Fm[1]);
+            ux[2] = Uc[2] - dtdx * (Fp[3] - Fm[3]);
+            ux[3] = Uc[3] - dtdx * (Fp[2] - Fm[2]);
+            W1 = W2;
+            dW1 = dW2;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Fm[k] = Fp[k];
+        };
+        // Y step of row i on its X-swept state (euler2d.hpp:176-236), RK2
+        // combine and wave speeds (stage 2), store
+        auto ystep = [&](int i, const double* ux) {
             double* po = out + (size_t)(i + 2) * snye;
             double U0[4];  // U^n at the output cell (stage 2 combine), issued early
             if constexpr (STAGE == 2) {
@@ -204,28 +227,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                     U0[3] = pu[3 * (size_t)vs];
                 }
             }
-            ring_put(i + 4, Un);
-            prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
-            prim4 dW2 = sub4(W2, W1);
-            prim4 s1 = slope4<LIM>(dW1, dW2);
-            double Fp[4];
-            face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
-            FL = face_plus(W1, s1);
-            // x_sweep_t update (euler2d.hpp:144-148)
-            const double ux0 = Uc[0] - dtdx * (Fp[0] - Fm[0]);
-            const double ux1 = Uc[1] - dtdx * (Fp[1] - Fm[1]);
-            const double ux2 = Uc[2] - dtdx * (Fp[3] - Fm[3]);
-            const double ux3 = Uc[3] - dtdx * (Fp[2] - Fm[2]);
-            W1 = W2;
-            dW1 = dW2;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                Fm[k] = Fp[k];
-                Un[k] = Unx[k];
-            }
-
-            // ---- Y-sweep of the X-swept row across lanes (euler2d.hpp:176-236)
-            prim4 Y = prim2d(ux0, ux1, ux2, ux3, false, a.eos);
+            prim4 Y = prim2d(ux[0], ux[1], ux[2], ux[3], false, a.eos);
             prim4 dl = sub4(Y, shfl_up4(Y));
             prim4 dr = shfl_down4(dl);
             prim4 sy = slope4<LIM>(dl, dr);
@@ -235,12 +237,12 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             double Gm[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) Gm[k] = lane_up(G[k]);
-            double o0 = ux0 - dtdy * (G[0] - Gm[0]);
-            double o1 = ux1 - dtdy * (G[3] - Gm[3]);
-            double o2 = ux2 - dtdy * (G[1] - Gm[1]);
-            double o3 = ux3 - dtdy * (G[2] - Gm[2]);
+            double o0 = ux[0] - dtdy * (G[0] - Gm[0]);
+            double o1 = ux[1] - dtdy * (G[3] - Gm[3]);
+            double o2 = ux[2] - dtdy * (G[1] - Gm[1]);
+            double o3 = ux[3] - dtdy * (G[2] - Gm[2]);
             if constexpr (STAGE == 2) {
-                // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2), in place of U^n
+                // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2)
                 if (store) {
                     o0 = 0.5 * (U0[0] + o0);
                     o1 = 0.5 * (U0[1] + o1);
@@ -259,6 +261,28 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                 po[vs] = o1;
                 po[2 * (size_t)vs] = o2;
                 po[3 * (size_t)vs] = o3;
+            }
+        };
+        if constexpr (STAGE == 1) {
+            // the march, unrolled by two so that the two row buffers
+            // alternate without register copies (a copy would wait for the
+            // load at once): each row load has two steps to arrive
+            for (int i = w.r0; i < w.r1; i += 2) {
+                double V[4];
+                xstep(i, V, UA, 2);
+                ystep(i, V);
+                if (i + 1 < w.r1) {
+                    xstep(i + 1, V, UB, 2);
+                    ystep(i + 1, V);
+                }
+            }
+        } else {
+            // stage 2 holds more state (U^n, the wave-speed maxima): one row
+            // buffer, loaded one step ahead, keeps it at 3 waves per SIMD
+            for (int i = w.r0; i < w.r1; ++i) {
+                double V[4];
+                xstep(i, V, UA, 1);
+                ystep(i, V);
             }
         }
         if constexpr (STAGE == 2) {
