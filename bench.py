@@ -190,6 +190,85 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
             "fd_max_rel_err_per_direction": float(f"{worst:.3e}")}
 
 
+def config5_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
+    """Config 5: Kelvin-Helmholtz 8192^2 periodic on 64x64 patches of 128^2,
+    `steps` RK2 steps with device dt (graph-replayed); then `nadj`
+    concurrent adjoint gradients of 256^2 instances of the config-4 setup
+    (each its own seed) in one batch handle, K = adj_steps, against the same
+    gradients one problem at a time."""
+    import numpy as np
+    import torch
+
+    from nest_amd import Sim2D, problems as P
+    from nest_amd.adjoint import Rollout, density_misfit
+
+    n, npch = 8192, 64
+    sim = Sim2D.mesh(n, n, npch, npch, bcx="periodic", bcy="periodic", mode=mode)
+    sim.upload(sim.scatter(P.kelvin_helmholtz(n)))
+    sim.fill_halos()
+    sim.start(cfl=0.4)
+    sim.advance(2, graph=True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(steps, graph=True)
+    sim.synchronize()
+    t_kh = time.perf_counter() - t0
+    st = sim.status()
+    sim.close()
+    kh = {"cells": n * n, "patches": npch * npch, "steps": steps,
+          "mcups": round(n * n * 2.0 * steps / t_kh / 1e6, 1), "ms_per_step": round(t_kh * 1e3 / steps, 3),
+          "t_reached": float(f"{st['t'][0]:.6g}")}
+
+    m = 256
+    setups = [P.adjoint_setup(m, seed=10 + 3 * b) for b in range(nadj)]
+    dt = min(s[2] for s in setups)
+    area = 1.0 / (m * m)
+    rb = Rollout.batch(nadj, m, m, nsteps=adj_steps, dt=dt, mode=mode)
+    fwd_cfg, adj_cfg = rb.tune([(m, m)] * nadj, log=log)
+    U0 = torch.from_numpy(np.stack([s[0] for s in setups])).cuda()
+    with torch.no_grad():
+        tgt = rb(torch.from_numpy(np.stack([s[1] for s in setups])).cuda())[:, 0].clone()
+
+    def batched():
+        U = U0.clone().requires_grad_(True)
+        density_misfit(rb(U), tgt, area).backward()
+        return U.grad
+
+    def forward():
+        with torch.no_grad():
+            density_misfit(rb(U0), tgt, area)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    t_b = timed(batched)
+    t_f = timed(forward)
+    rb.close()
+    rs = Rollout.mesh(m, m, nsteps=adj_steps, dt=dt, mode=mode)
+    rs.tune([(m, m)], log=log)
+    nseq = 8  # single-problem gradients timed (the rate is per gradient)
+
+    def sequential():
+        for b in range(nseq):
+            u = U0[b].clone().requires_grad_(True)
+            density_misfit(rs(u), tgt[b], area).backward()
+
+    t_s = timed(sequential) / nseq
+    rs.close()
+    return {"kh": kh, "batched_adjoints": {
+        "problems": nadj, "side": m, "steps": adj_steps,
+        "gradient_ms": round(t_b * 1e3, 2), "forward_ms": round(t_f * 1e3, 2),
+        "adjoint_fwd_cost": round(t_b / t_f, 3),
+        "gradients_per_s": round(nadj / t_b, 2),
+        "single_problem_gradient_ms": round(t_s * 1e3, 2),
+        "batching_speedup": round(t_s * nadj / t_b, 2)}}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic_stage2d_sod4096.json")
     if os.path.exists(path):
@@ -211,6 +290,7 @@ def main():
     ap.add_argument("--per-patch-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=100, help="config-3 leg; 0 skips it")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="config-4 leg; 0 skips it")
+    ap.add_argument("--config5-steps", type=int, default=100, help="config-5 leg (KH 8192^2 + batched adjoints); 0 skips it")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling aid: one plain device copy of this many MiB after the legs")
     args = ap.parse_args()
@@ -301,6 +381,11 @@ def main():
         log("bench: adjoint (config 4)")
         adj = adjoint_leg(args.mode, K=args.adjoint_steps)
 
+    c5 = None
+    if args.config5_steps > 0:
+        log("bench: config 5 (KH 8192^2, 64 batched 256^2 adjoints)")
+        c5 = config5_leg(args.mode, steps=args.config5_steps)
+
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -342,6 +427,7 @@ def main():
         "per_patch_value": pp,
         "ensemble": ens,
         "adjoint": adj,
+        "config5": c5,
         "roofline": {"bound": "hbm", "kernel": "stage2d_kernel<minmod, 2>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),

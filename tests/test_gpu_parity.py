@@ -288,6 +288,38 @@ def test_config2_full_size_one_step(oracle):
     assert np.array_equal(got, U)
 
 
+def test_config5_kh_full_size_one_step(oracle):
+    """Config 5's mesh at full size (Kelvin-Helmholtz 8192^2, periodic, 64x64
+    patches of 128^2): one RK2 step with the CFL dt, bitwise vs the oracle in
+    exact numerics and within 1e-12 in fast numerics; the periodic totals of
+    mass and energy are conserved to round-off."""
+    n, npt = 8192, 64
+    print_case("config5 KH full size")
+    m = MeshSpec(n, n, npt, npt, bcx="periodic", bcy="periodic")
+    G = P.kelvin_helmholtz(n)
+    U = m.to_patches(G)
+    del G
+    oracle.mesh_fill(U, m)
+    dt = oracle.mesh_dt(U, m, 1.0 / n, 1.0 / n, 0.4)
+    ref = U.copy()
+    oracle.mesh_rk2_step(ref, m, dt, 1.0 / n, 1.0 / n)
+    want = m.to_global(ref)
+    del ref
+    for mode in ("exact", "fast"):
+        sim = Sim2D.mesh(n, n, npt, npt, bcx="periodic", bcy="periodic", mode=mode)
+        sim.upload(U)
+        sim.rk2_step(dt)
+        got = m.to_global(sim.download())
+        sim.close()
+        if mode == "exact":
+            assert np.array_equal(got, want)
+        else:
+            assert rel_linf_vars(got, want) <= TOL_FAST
+        for k in (0, 3):
+            tot0 = np.sum(m.to_global(U)[k], dtype=np.float64)
+            assert abs(np.sum(got[k]) - tot0) <= 1e-12 * abs(tot0)
+
+
 def test_config3_ensemble_batched_equals_per_patch_and_oracle(oracle):
     """Config 3 at full size (512 problems, 64..512 cells a side): 3 batched
     device-dt steps equal 3 steps of the plain per-patch path bitwise, and a
