@@ -197,8 +197,10 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
 // ---------------------------------------------------------------------------
 // X kernel
 // ---------------------------------------------------------------------------
+// Two waves per SIMD: the register budget (256 per lane) holds the march's
+// carried state and one face reverse without spilling.
 template <int LIM>
-__global__ __launch_bounds__(256) void adj_x_kernel(adj_args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void adj_x_kernel(adj_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
@@ -247,13 +249,16 @@ __global__ __launch_bounds__(256) void adj_x_kernel(adj_args a) {
             const int gr = pd.x0 + t;
             return ring || (gr >= 0 && gr < G.NX);
         };
-        auto emit = [&](int row, const prim4& Wp, const prim4& Wbar, const double* Vbar) {
+        // out(row) = Vb(row) + (dW/dU)^T Wbar (+ h H(row)); Vb(row) is re-read
+        // here (it was loaded two steps earlier, so it is cache-resident)
+        // rather than carried in registers
+        auto emit = [&](int row, const prim4& Wp, const prim4& Wbar) {
             double Ub[4];
             prim2d_rev(Wp, true, Wbar, a.eos, Ub);
             const long long o = col + (long long)(row + 2) * snye;
             double r[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = Vbar[k] + Ub[k];
+            for (int k = 0; k < 4; ++k) r[k] = a.Vb[o + k * (long long)svs] + Ub[k];
             if (a.H) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) r[k] += a.h * a.H[o + k * (long long)svs];
@@ -270,9 +275,8 @@ __global__ __launch_bounds__(256) void adj_x_kernel(adj_args a) {
         load_pos(a.U, tb, u);
         prim4 W3 = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
         prim4 W2 = W3, W1 = W3;
-        double Vb3[4] = {0.0, 0.0, 0.0, 0.0}, Vb2[4] = {0.0, 0.0, 0.0, 0.0};
+        double Vb2[4] = {0.0, 0.0, 0.0, 0.0};
         prim4 Wb3 = zero4(), Wb2 = zero4(), sb2 = zero4(), acc = zero4();
-        double Vhold[4] = {0.0, 0.0, 0.0, 0.0};
         for (int t = tb + 1; t < te; ++t) {
             load_pos(a.U, t, u);
             const prim4 Wt = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
@@ -303,26 +307,21 @@ __global__ __launch_bounds__(256) void adj_x_kernel(adj_args a) {
             if (lo && p < 0) {
                 acc = add4(acc, Wb3);
             } else if (lo && p == 0) {
-                emit(0, W3, add4(Wb3, acc), Vb3);
+                emit(0, W3, add4(Wb3, acc));
                 acc = zero4();
             } else if (hi && p == w.r1 - 1) {
                 acc = Wb3;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) Vhold[k] = Vb3[k];
             } else if (hi && p >= w.r1) {
                 acc = add4(acc, Wb3);
-                if (p == w.r1 + 1) emit(w.r1 - 1, W3, acc, Vhold);  // W3: ghost copy of row NX-1
+                if (p == w.r1 + 1) emit(w.r1 - 1, W3, acc);  // W3: ghost copy of row NX-1
             } else if (p >= w.r0 && p < w.r1) {
-                emit(p, W3, Wb3, Vb3);
+                emit(p, W3, Wb3);
             }
             W3 = W2;
             W2 = W1;
             W1 = Wt;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                Vb3[k] = Vb2[k];
-                Vb2[k] = Vb1[k];
-            }
+            for (int k = 0; k < 4; ++k) Vb2[k] = Vb1[k];
             Wb3 = Wb2;
             Wb2 = Wb1;
             sb2 = sb1;
