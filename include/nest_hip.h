@@ -206,8 +206,9 @@ int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double
 int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval);
 /* Reverse of the last rollout: Ubar (in: cotangent of the final state, out:
    cotangent of the initial state), recomputing each checkpoint interval's
-   states (and their stage-1 states) from its checkpoint. Memory:
-   ceil(K/m) + 2m - 1 + 3 states. */
+   states (and their stage-1 states) from its checkpoint; the recompute of
+   one interval runs on an auxiliary stream while the next is reversed.
+   Memory: ceil(K/m) + 2 (2m - 1) + 3 states. */
 int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar);
 int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval, double* dt);
 /* advance() without a graph, with HIP events recorded on the handle's stream

@@ -51,6 +51,25 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
+    // per-wave LDS ring of the last 4 loaded rows: the centre row of the X
+    // update comes from here instead of a second read from memory (as in the
+    // forward stage kernel); each lane touches only its own column
+    __shared__ double ring_all[4 * 4 * 4 * 64];
+    double* const ring = ring_all + (threadIdx.x >> 6) * (4 * 4 * 64) + lane;
+    auto ring_put = [&](int e, const double* u) {
+        double* q = ring + (e & 3) * (4 * 64);
+        q[0] = u[0];
+        q[64] = u[1];
+        q[128] = u[2];
+        q[192] = u[3];
+    };
+    auto ring_get = [&](int e, double* u) {
+        const double* q = ring + (e & 3) * (4 * 64);
+        u[0] = q[0];
+        u[1] = q[64];
+        u[2] = q[128];
+        u[3] = q[192];
+    };
     for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
@@ -102,8 +121,10 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
             load_row(w.r0 + 1, A);
             prim4 Wm1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
             load_row(w.r0 + 2, A);
+            ring_put(w.r0 + 2, A);
             prim4 W0 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
             load_row(w.r0 + 3, A);
+            ring_put(w.r0 + 3, A);
             W1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
             prim4 d0 = sub4(W0, Wm1);
             dW1 = sub4(W1, W0);
@@ -117,7 +138,8 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
         for (int i = w.r0; i < w.r1; ++i) {
             double Unx[4], Uc[4];
             load_row(min(i + 5, pd.nx + 3), Unx);
-            load_row(i + 2, Uc);
+            ring_get(i + 2, Uc);
+            ring_put(i + 4, Un);
             const double* pg = a.G + col + (long long)(i + 2) * snye;
             double A[4];
             A[0] = pg[0];
@@ -199,7 +221,9 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
 // ---------------------------------------------------------------------------
 // Two waves per SIMD: the register budget (256 per lane) holds the march's
 // carried state and one face reverse without spilling.
-template <int LIM>
+// OUTX: some problem of the launch has outflow x boundaries (the ghost-row
+// fold is compiled in only then).
+template <int LIM, bool OUTX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void adj_x_kernel(adj_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
@@ -220,8 +244,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
         const int svs = (qd.nx + 4) * snye;
         const long long col = qd.off + lc;
         const bool ring = G.bcx == 0;
-        const bool lo = !ring && pd.x0 + w.r0 == 0;     // outflow end at row 0
-        const bool hi = !ring && pd.x0 + w.r1 == G.NX;  // outflow end at row NX-1
+        const bool lo = OUTX && !ring && pd.x0 + w.r0 == 0;     // outflow end at row 0
+        const bool hi = OUTX && !ring && pd.x0 + w.r1 == G.NX;  // outflow end at row NX-1
 
         // value of buffer B at position t (local row of the home patch, any
         // integer; rows outside the patch resolve through the boundary map)
@@ -365,12 +389,19 @@ void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s
         default: hipLaunchKernelGGL(adj_y_kernel<LIM_MINMOD>, grid, block, 0, s, a); break;
     }
 }
-void launch_adj_x(int lim, const adj_args& a, int grid, int block, hipStream_t s) {
+template <bool OUTX>
+void launch_adj_x_bc(int lim, const adj_args& a, int grid, int block, hipStream_t s) {
     switch (lim) {
-        case LIM_MC: hipLaunchKernelGGL(adj_x_kernel<LIM_MC>, grid, block, 0, s, a); break;
-        case LIM_NONE: hipLaunchKernelGGL(adj_x_kernel<LIM_NONE>, grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL(adj_x_kernel<LIM_MINMOD>, grid, block, 0, s, a); break;
+        case LIM_MC: hipLaunchKernelGGL((adj_x_kernel<LIM_MC, OUTX>), grid, block, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((adj_x_kernel<LIM_NONE, OUTX>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((adj_x_kernel<LIM_MINMOD, OUTX>), grid, block, 0, s, a); break;
     }
+}
+void launch_adj_x(int lim, const adj_args& a, int grid, int block, hipStream_t s) {
+    if (a.outflow_x)
+        launch_adj_x_bc<true>(lim, a, grid, block, s);
+    else
+        launch_adj_x_bc<false>(lim, a, grid, block, s);
 }
 
 }  // namespace NEST_NS

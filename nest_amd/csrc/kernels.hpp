@@ -63,6 +63,7 @@ struct adj_args {
     double dt;
     const nest_prob_geom* geoms;
     const int* itab;
+    int outflow_x;  // some problem has outflow x boundaries
 };
 
 // arguments of the 1D kernels (one problem, one patch)

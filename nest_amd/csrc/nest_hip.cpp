@@ -159,8 +159,12 @@ struct nest_sim2d {
     long long gtotal = 0;
     long long* goff_d = nullptr;
     std::vector<double*> ckpt;  // rollout checkpoints (every roll_m steps)
-    std::vector<double*> seg;   // recomputed states of one checkpoint interval
-    std::vector<double*> seg1;  // their stage-1 states
+    // recomputed states of a checkpoint interval and their stage-1 states,
+    // two sets: the recompute of one interval (auxiliary stream) overlaps the
+    // reverse sweep of the next (handle's stream)
+    std::vector<double*> seg[2], seg1[2];
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     int roll_K = 0, roll_m = 0;
     double roll_dt = 0.0;
 
@@ -179,10 +183,14 @@ struct nest_sim2d {
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
                         (void*)goff_d})
             if (p) (void)hipFree(p);
-        for (auto* v : {&ckpt, &seg, &seg1}) {
+        for (auto* v : {&ckpt, &seg[0], &seg[1], &seg1[0], &seg1[1]}) {
             for (double* p : *v) (void)hipFree(p);
             v->clear();
         }
+        for (hipEvent_t* e : {&ev_start, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
+            if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+        if (aux_stream) (void)hipStreamDestroy(aux_stream);
+        aux_stream = nullptr;
         aitems_d[0] = aitems_d[1] = nullptr;
         adj_U1 = adj_Vb = adj_Ub1 = nullptr;
         goff_d = nullptr;
@@ -348,14 +356,20 @@ struct nest_sim2d {
     // forward stage st on an arbitrary state buffer X (stage 1: X -> X1;
     // stage 2: X <- 0.5 (X + S(X1)) in place), host dt
     // (stage 2 with Xo: Xo <- 0.5 (X + S(X1)), X left intact)
-    void stage_on(int st, double* X, double* X1, double dt, double* Xo = nullptr) {
+    void stage_on(int st, double* X, double* X1, double dt, double* Xo = nullptr, hipStream_t on = nullptr) {
         const int n = (int)gitems_h[st - 1].size();
         const int w = gwpb[st - 1];
         stage_args a = args(dt, 0, n, st);
         a.U = X;
         a.U1 = X1;
         a.Uo = Xo;
-        L().stage2d(st, lim, a, grid_for(n, w), 64 * w, stream);
+        L().stage2d(st, lim, a, grid_for(n, w), 64 * w, on ? on : stream);
+    }
+    int ensure_aux_stream() {
+        if (!aux_stream) NEST_HIP_TRY(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&ev_start, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
+            if (!*e) NEST_HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        return NEST_OK;
     }
     int ensure_adjoint_scratch() {
         for (double** p : {&adj_U1, &adj_Vb, &adj_Ub1})
@@ -390,6 +404,8 @@ struct nest_sim2d {
         a.dt = dt;
         a.geoms = geoms_d;
         a.itab = itab_d;
+        a.outflow_x = 0;
+        for (const auto& g : tb.geoms) a.outflow_x |= (g.bcx != 0);
         for (int k = 0; k < 2; ++k) {
             a.items = aitems_d[k];
             a.nitems = (int)aitems_h[k].size();
@@ -885,19 +901,37 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     if (int rc = s->ensure_adjoint_scratch()) return rc;
     const int K = s->roll_K, m = s->roll_m;
     const double dt = s->roll_dt;
-    if (int rc = s->grow(s->seg, (size_t)std::max(0, m - 1))) return rc;
-    if (int rc = s->grow(s->seg1, (size_t)m)) return rc;
     const int nck = (K + m - 1) / m;
-    for (int c = nck - 1; c >= 0; --c) {
-        const int len = std::min(m, K - c * m);
-        // states of steps c*m .. c*m + len - 1 (the checkpoint, then
-        // recomputed) and the stage-1 state of each, kept for the reverse
-        auto state = [&](int i) { return i == 0 ? s->ckpt[c] : s->seg[i - 1]; };
+    if (int rc = s->ensure_aux_stream()) return rc;
+    for (int set = 0; set < (nck > 1 ? 2 : 1); ++set) {
+        if (int rc = s->grow(s->seg[set], (size_t)std::max(0, m - 1))) return rc;
+        if (int rc = s->grow(s->seg1[set], (size_t)m)) return rc;
+    }
+    // states of steps c*m .. c*m + len - 1 of interval c: the checkpoint, then
+    // recomputed into buffer set c % 2, with the stage-1 state of each
+    auto state = [&](int c, int i) { return i == 0 ? s->ckpt[c] : s->seg[c & 1][i - 1]; };
+    auto recompute = [&](int c) {
+        const int len = std::min(m, K - c * m), set = c & 1;
         for (int i = 0; i < len; ++i) {
-            s->stage_on(1, state(i), s->seg1[i], dt);
-            if (i + 1 < len) s->stage_on(2, state(i), s->seg1[i], dt, state(i + 1));
+            s->stage_on(1, state(c, i), s->seg1[set][i], dt, nullptr, s->aux_stream);
+            if (i + 1 < len) s->stage_on(2, state(c, i), s->seg1[set][i], dt, state(c, i + 1), s->aux_stream);
         }
-        for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(i), s->seg1[i], Ubar, dt);
+        return hipEventRecord(s->ev_rec[set], s->aux_stream);
+    };
+    // everything enqueued before (the rollout and its checkpoints) comes first
+    NEST_HIP_TRY(hipEventRecord(s->ev_start, s->stream));
+    NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_start, 0));
+    NEST_HIP_TRY(recompute(nck - 1));
+    for (int c = nck - 1; c >= 0; --c) {
+        const int set = c & 1, len = std::min(m, K - c * m);
+        NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec[set], 0));
+        if (c >= 1) {
+            // interval c-1 reuses the set that the reverse of interval c+1 read
+            if (c + 1 <= nck - 1) NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_rev[(c - 1) & 1], 0));
+            NEST_HIP_TRY(recompute(c - 1));
+        }
+        for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
+        NEST_HIP_TRY(hipEventRecord(s->ev_rev[set], s->stream));
     }
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
