@@ -23,13 +23,21 @@
 #include "device_numerics.hpp"
 #include "kernels.hpp"
 
-// Minimum waves per SIMD requested for the stage kernel (register budget
-// 512/N per lane); unset = the compiler's choice.
-#ifdef NEST_STAGE_WAVES
-#define NEST_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NEST_STAGE_WAVES, 8)))
-#else
-#define NEST_STAGE_OCCUPANCY
+// Minimum waves per SIMD requested for stage 1 / stage 2 of the stage kernel
+// (register budget 512/N per lane); 1 = the compiler's choice.
+#ifndef NEST_STAGE1_WAVES
+#define NEST_STAGE1_WAVES 1
 #endif
+#ifndef NEST_STAGE2_WAVES
+#define NEST_STAGE2_WAVES 1
+#endif
+// Stage 1's strip march unrolled by two with alternating row buffers (1) or
+// one row buffer loaded a step ahead (0).
+#ifndef NEST_STAGE1_UNROLL
+#define NEST_STAGE1_UNROLL 1
+#endif
+#define NEST_STAGE_OCCUPANCY \
+    __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? NEST_STAGE1_WAVES : NEST_STAGE2_WAVES, 8)))
 
 namespace NEST_NS {
 
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
         unsigned long long smx = 0, smy = 0;
         double UA[4], UB[4];  // rows e = r0+4 (and r0+5, stage 1) for the first X steps
         load_row(w.r0 + 4, UA);
-        if constexpr (STAGE == 1) load_row(min(w.r0 + 5, pd.nx + 3), UB);
+        if constexpr (STAGE == 1 && NEST_STAGE1_UNROLL) load_row(min(w.r0 + 5, pd.nx + 3), UB);
         // X step of row j: consumes the window and the row loaded for it (Ub,
         // extended row j+4), refills Ub with the row two steps ahead (two row
         // buffers alternate, so each load has two steps to arrive), writes
@@ -263,7 +271,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                 po[3 * (size_t)vs] = o3;
             }
         };
-        if constexpr (STAGE == 1) {
+        if constexpr (STAGE == 1 && NEST_STAGE1_UNROLL) {
             // the march, unrolled by two so that the two row buffers
             // alternate without register copies (a copy would wait for the
             // load at once): each row load has two steps to arrive
@@ -277,8 +285,8 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                 }
             }
         } else {
-            // stage 2 holds more state (U^n, the wave-speed maxima): one row
-            // buffer, loaded one step ahead, keeps it at 3 waves per SIMD
+            // one row buffer, loaded one step ahead (stage 2 holds more state,
+            // U^n and the wave-speed maxima: this keeps it at 3 waves per SIMD)
             for (int i = w.r0; i < w.r1; ++i) {
                 double V[4];
                 xstep(i, V, UA, 1);
