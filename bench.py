@@ -99,7 +99,7 @@ def ensemble_leg(steps: int, mode: str):
     return {"problems": 512, "cells": int(cells), "steps": steps, "batched_mcups": round(batched, 1),
             "per_patch_mcups": round(per_patch, 1), "speedup": round(batched / per_patch, 2),
             "per_patch_steps_timed": pp_steps,
-            "tiling": {k: v for k, v in cfg.items() if k in ("1", "2")}}
+            "tiling": {k: v for k, v in cfg.items() if k in ("1", "2", "persistent")}}
 
 
 def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps: int = 3):

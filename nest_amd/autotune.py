@@ -4,7 +4,9 @@ Per stage (1 and 2) it picks, by measurement on the live problem set:
   * rows per work item for each problem size class (the tile height of the
     strip march: short tiles give more waves and a finer tail, long tiles
     amortise the 4-row warm-up of the X window), and
-  * waves per block of the launch.
+  * waves per block of the launch, and
+  * whether the launch is persistent (grid capped at the resident waves,
+    each wave looping over the device work list) or one wave per item.
 Search: coordinate descent from the defaults, each candidate timed over a
 few device-dt steps with HIP events on the handle's stream (the same timing
 path as bench.py). Results are cached in a JSON file keyed by device name
@@ -52,7 +54,9 @@ def _time(sim, steps):
 
 
 def apply(sim, classes, cfg) -> None:
-    """Set a tuned configuration {stage: {"rows": [per class], "waves": w}}."""
+    """Set a tuned configuration {stage: {"rows": [per class], "waves": w},
+    "persistent": 0|1}."""
+    sim.set_tiling(0, 0, cfg.get("persistent", 0))
     for st in (1, 2):
         c = cfg[str(st)]
         sim.set_stage_tiling(st, 0, c["waves"])
@@ -163,6 +167,15 @@ def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = N
             trials.append((st, "waves", w, t))
             if t < best:
                 best, cfg = t, trial
+    # persistent grid (capped at the resident waves, looping over the work
+    # list) against one wave per work item
+    trial = json.loads(json.dumps(cfg))
+    trial["persistent"] = 1
+    apply(sim, classes, trial)
+    t = sum(_time(sim, steps))
+    trials.append(("persistent", 1, t))
+    if t < best:
+        best, cfg = t, trial
     apply(sim, classes, cfg)
     cfg["ms_per_step"] = round(best, 5)
     if log:
