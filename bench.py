@@ -295,8 +295,8 @@ def main():
                     help="profiling aid: one plain device copy of this many MiB after the legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (single-GPU tier); "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print("bench.py: measures one GPU only; sharding the path over several GPUs is "
+              "designed but not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

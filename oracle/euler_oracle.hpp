@@ -2,7 +2,8 @@
 //
 // A CPU restatement of nest's Euler finite-volume numerics, written from the
 // reference's published algorithm, used only by tests/, __graft_entry__.smoke()
-// and bench.py's cpu_baseline leg as the checker. It is never linked into the
+// and bench.py's cpu_baseline leg, as the reference results that the HIP path is
+// compared with. It is never linked into the
 // product library (nest_amd/libnest_hip.so) and nothing in the product path
 // calls it.
 //

@@ -4,7 +4,7 @@ Loads ``oracle/liboracle.so`` (the CPU restatement of nest's Euler numerics,
 built by ``oracle/Makefile``) and, when present, ``oracle/_ref/libnest_ref.so``
 (the reference's own headers driven by ``oracle/ref_harness.cpp``). Only
 ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg may
-import this module, and only as the checker / baseline — never as the thing
+import this module, and only as the comparison reference / baseline — never as the thing
 measured or shipped.
 
 All arrays are float64 numpy arrays in field_t SoA layout (variable-major,

@@ -2,7 +2,7 @@
 
 Exact mode must be bitwise equal to the oracle (IEEE division and sqrt, no
 FMA contraction on either side). Fast mode must agree to 1e-12 relative
-L-infinity per step; measured deviations are ~1e-15.
+L-infinity per step (the required bound); measured deviations are ~1e-15.
 """
 import numpy as np
 import pytest
