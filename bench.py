@@ -63,7 +63,40 @@ def cpu_baseline(G, budget_s: float):
     _ = np
     return {"value": round(val, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} RK2 step(s) of the same 4096^2 16x16-patch Sod workload (dt from the "
-                      f"state each step), {el:.1f} s wall on {threads} OpenMP threads"}
+                      f"state each step), {el:.1f} s wall on {threads} OpenMP threads "
+                      f"of {os.cpu_count()} visible ({cpu_model()})"}
+
+
+def copy_peak_gbs(gib: int = 2, reps: int = 5) -> float:
+    """Measured device-copy bandwidth: median of `reps` torch D2D copies of
+    `gib` GiB (read + write bytes / time, HIP events)."""
+    import torch
+
+    n = gib * (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b) * 1e-3)
+    del src, dst
+    return 2.0 * n * 8 / sorted(ts)[reps // 2] / 1e9
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def ensemble_leg(steps: int, mode: str):
@@ -166,13 +199,17 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
     ro.interval = 0
     Ug = torch.from_numpy(P.adjoint_generic_state(n)).cuda()
     g = gradient(Ug)
-    d_adj, d_fd = [], []
+    scan = (1e-5, 1e-6, 1e-7, 1e-8, 1e-9)
+    d_adj, d_fd = [], {e: [] for e in scan}
     with torch.no_grad():
         for k in range(ndir):
             v = torch.from_numpy(P.smooth_direction(n, 200 + k)).cuda()
             d_adj.append(float(torch.sum(g * v)))
-            d_fd.append((float(loss(Ug + eps * v)) - float(loss(Ug - eps * v))) / (2 * eps))
-    a, f = np.array(d_adj), np.array(d_fd)
+            for e in scan:
+                d_fd[e].append((float(loss(Ug + e * v)) - float(loss(Ug - e * v))) / (2 * e))
+    a = np.array(d_adj)
+    scan_err = {f"{e:.0e}": float(f"{np.linalg.norm(np.array(d_fd[e]) - a) / np.linalg.norm(a):.3e}") for e in scan}
+    f = np.array(d_fd[eps])
     proj_err = float(np.linalg.norm(f - a) / np.linalg.norm(a))
     worst = float(np.max(np.abs(f - a) / np.abs(a)))
     ro.close()
@@ -186,7 +223,9 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
             "tiling": {"forward": {k: v for k, v in fwd_cfg.items() if k in ("1", "2")},
                        "adjoint": {k: v for k, v in adj_cfg.items() if k != "ms_per_vjp"}},
             "fd_point": "config-4 state + seeded 1e-3 smooth perturbation of rho, u, v, p",
+            "dt": float(dt).hex(),
             "fd_directions": ndir, "fd_eps": eps, "fd_projected_rel_l2_err": float(f"{proj_err:.3e}"),
+            "fd_eps_scan": scan_err,
             "fd_max_rel_err_per_direction": float(f"{worst:.3e}")}
 
 
@@ -401,6 +440,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    copy_gbs = copy_peak_gbs()
     traffic = load_traffic()
     line = {
         "metric": METRIC,
@@ -431,6 +471,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "stage2d_kernel<minmod, 2>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),
+                     "measured_copy_peak": round(copy_gbs, 1), "frac_vs_copy_peak": round(achieved / copy_gbs, 4),
                      "traffic": (traffic or {}).get("stage2_bytes_per_launch")},
         "cpu_baseline": cpu,
     }

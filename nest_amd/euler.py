@@ -63,6 +63,13 @@ class Solver1D:
     def rk2_step(self, dt: float) -> None:
         check(self.L.nest_hip_solver1d_rk2_step(self.h, dt))
 
+    def write_dat(self, directory: str, time: float, step: int, x_min: float = 0.0, x_max: float = 1.0,
+                  gamma: float = 1.4) -> str:
+        """sod_NNNN.dat of the current state, byte-compatible with the
+        reference's sod1d writer (nest_amd.dat.write_sod1d)."""
+        from . import dat
+        return dat.write_sod1d(directory, self.download().reshape(3, -1)[:, 2:-2], time, step, x_min, x_max, gamma)
+
     def run(self, t_final: float, cfl: float) -> int:
         k = C.c_int()
         check(self.L.nest_hip_solver1d_run(self.h, t_final, cfl, C.byref(k)))
@@ -165,6 +172,19 @@ class Sim2D:
         a, b = C.c_void_p(), C.c_void_p()
         check(self.L.nest_hip_sim2d_device_ptrs(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def write_dat(self, directory: str, time: float, step: int, x_min: float = 0.0, x_max: float = 1.0,
+                  y_min: float = 0.0, y_max: float = 0.1, gamma: float = 1.4, xslice: bool = True):
+        """sod2d_NNNN.dat (and sod2d_xslice_NNNN.dat) of a mesh's current
+        state, byte-compatible with the reference's sod2d writers
+        (nest_amd.dat); returns the paths written."""
+        from . import dat
+        nx, ny, npx, npy = self.grid
+        G = self.gather(self.download())
+        paths = [dat.write_sod2d(directory, G, time, step, npx, npy, x_min, x_max, y_min, y_max, gamma)]
+        if xslice:
+            paths.append(dat.write_sod2d_xslice(directory, G, time, step, x_min, x_max, gamma))
+        return paths
 
     def bind(self, dev_U: int | None, dev_U1: int | None = None) -> None:
         check(self.L.nest_hip_sim2d_bind(self.h, C.c_void_p(dev_U or 0), C.c_void_p(dev_U1 or 0)))
