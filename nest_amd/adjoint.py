@@ -60,7 +60,9 @@ class Rollout:
     def tune(self, sizes, log=None):
         """Autotune the forward stage and adjoint kernels' tiling on this
         handle (results cached per device and problem-size signature). The
-        handle's state is overwritten (every forward call reloads it)."""
+        handle's state and recorded rollout are overwritten (every forward
+        call reloads the state); tune before a forward, not between a forward
+        and its backward."""
         from . import autotune
         self.sim.start(cfl=0.4)
         fwd = autotune.tune(self.sim, sizes, log=log)

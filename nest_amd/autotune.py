@@ -14,7 +14,8 @@ and the problem set's size-class histogram, so a later run on the same kind
 of input starts from the tuned point.
 
 tune_adjoint() does the same for the two adjoint kernels (rows per work item
-and waves per block of each), timing whole RK2-step vector-Jacobian products.
+and waves per block of each), timing a short checkpointed rollout and its
+reverse sweep (the path the differentiable rollouts take).
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ import time
 import numpy as np
 
 ROWS = (8, 12, 16, 24, 32, 48, 64, 96, 128)
-ADJ_ROWS = (8, 12, 16, 24, 32, 48)
+ADJ_ROWS = (6, 8, 12, 16, 24, 32, 48)
 WAVES = (1, 2, 4)
 
 
@@ -83,28 +84,34 @@ def _store(path, table):
 def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 3, cache: str | None = None, log=None):
     """Tune the adjoint kernels' tiling of `sim` in place: coordinate descent
     over rows and waves per block of the Y and X kernels, each candidate
-    timed over `reps` RK2-step VJPs (the handle's own state as U0 and its
-    stage scratch as the cotangent, so no extra memory). Returns
-    {"rows_y", "waves_y", "rows_x", "waves_x"}."""
+    timed over `reps` runs of a 4-step rollout (checkpoints every 2 steps)
+    and its reverse sweep, from the handle's own state (advanced by each
+    run) with its stage scratch as the cotangent, so no extra memory.
+    Returns {"rows_y", "waves_y", "rows_x", "waves_x", "ms_per_step"}."""
     classes = size_classes(sizes)
-    key = "adjoint|" + signature(sizes, classes, arch)
+    key = "adjoint-rollout|" + signature(sizes, classes, arch)
     path = _cache_path(cache)
     table = _load(path)
     if key in table:
         c = table[key]
         sim.set_adjoint_tiling(c["rows_y"], c["waves_y"], c["rows_x"], c["waves_x"])
         return c
-    U, U1 = sim.device_ptrs()
+    _, U1 = sim.device_ptrs()
+    nsteps = 4
+
+    def once():
+        sim.rollout(nsteps, dt, 2)
+        sim.rollout_adjoint(U1)
 
     def run(c):
         sim.set_adjoint_tiling(c["rows_y"], c["waves_y"], c["rows_x"], c["waves_x"])
-        sim.rk2_vjp(U, U1, dt)
+        once()
         sim.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            sim.rk2_vjp(U, U1, dt)
+            once()
         sim.synchronize()
-        return (time.perf_counter() - t0) / reps
+        return (time.perf_counter() - t0) / (reps * nsteps)
 
     cfg = {"rows_y": 32, "waves_y": 4, "rows_x": 32, "waves_x": 4}
     best = run(cfg)
@@ -120,7 +127,7 @@ def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 3, cac
             if t < best:
                 best, cfg = t, trial
     sim.set_adjoint_tiling(cfg["rows_y"], cfg["waves_y"], cfg["rows_x"], cfg["waves_x"])
-    cfg["ms_per_vjp"] = round(best * 1e3, 4)
+    cfg["ms_per_step"] = round(best * 1e3, 4)
     if log:
         log(f"autotune {key}: {cfg} ({ntrials} trials)")
     table[key] = cfg

@@ -12,7 +12,8 @@
 // Y kernel (one wave per strip of 64 extended columns, lanes 3..60 own
 // output, marching interior rows like the forward stage kernel):
 //   * recomputes the X-swept row V(i) with the forward's sliding register
-//     window (the X-sweep of the halo columns is what the fill would copy);
+//     window (the X-sweep of the halo columns is what the fill would copy),
+//     one row ahead of the Y reverse (software pipeline);
 //   * Y-sweep reverse across lanes: cotangent of each y-face flux from the
 //     stage-output cotangent, reverse HLLE + passive flux, slopes, limiter
 //     and primitives, all neighbour terms moved by wave shuffles;
@@ -26,6 +27,15 @@
 // then emitted as out = Vb + (dW/dU)^T Wb (+ h*H).
 #include "adjoint_numerics.hpp"
 #include "kernels.hpp"
+
+// Minimum waves per SIMD requested for the Y and X kernels (register budget
+// 512/N per lane).
+#ifndef NEST_ADJY_WAVES
+#define NEST_ADJY_WAVES 1
+#endif
+#ifndef NEST_ADJX_WAVES
+#define NEST_ADJX_WAVES 2
+#endif
 
 namespace NEST_NS {
 
@@ -47,7 +57,8 @@ __device__ __forceinline__ prim4 dn1(const prim4& a) {
 // Y kernel
 // ---------------------------------------------------------------------------
 template <int LIM>
-__global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_WAVES, 8))) void adj_y_kernel(
+    adj_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
@@ -90,6 +101,71 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
         const bool real = ring || (gcol >= 0 && gcol < G.NY);  // outflow ghosts are not outputs
         const bool own = lane >= 3 && lane < 61 && gcol < G.NY;
         const bool edge = !ring && (gs == 0 || gs + 58 >= G.NY);  // wave-uniform
+
+        // interior row i of buffer B at this lane's column
+        auto load_int = [&](const double* B, int i, double* u) {
+            const double* p = B + col + (long long)(i + 2) * snye;
+            u[0] = p[0];
+            u[1] = p[svs];
+            u[2] = p[2 * (long long)svs];
+            u[3] = p[3 * (long long)svs];
+        };
+        // Y-sweep reverse of row i across lanes, from its X-swept state v and
+        // the stage-output cotangent A (unscaled); writes Vb(i)
+        auto yrev = [&](int i, const double* v, double* A) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) A[k] = real ? a.g * A[k] : 0.0;
+            const prim4 Y = prim2d(v[0], v[1], v[2], v[3], false, a.eos);
+            const prim4 dl = sub4(Y, up1(Y));
+            const prim4 dr = dn1(dl);
+            const prim4 sy = slope4<LIM>(dl, dr);
+            const limsel4<LIM> cf = limiter_sel4<LIM>(dl, dr);
+            const prim4 Lf = face_plus(Y, sy);
+            const prim4 Rf = dn1(face_minus(Y, sy));
+            // face between this lane and the next: flux cotangent from the
+            // output cotangents of the two cells (y_sweep_t update order:
+            // rho <- G0, m_u <- G3, m_v <- G1, E <- G2)
+            double An[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) An[k] = lane_down(A[k]);
+            double Fb[4];
+            Fb[0] = dtdy * (An[0] - A[0]);
+            Fb[1] = dtdy * (An[2] - A[2]);
+            Fb[2] = dtdy * (An[3] - A[3]);
+            Fb[3] = dtdy * (An[1] - A[1]);
+            // every lane reverses its face (branch-free); lanes 0, 62, 63
+            // have no face of the strip and drop theirs
+            prim4 Lb = zero4(), Rb = zero4();
+            face_flux_rev(Lf, Rf, a.eos, Fb, Lb, Rb);
+            const bool has_face = lane >= 1 && lane <= 61;
+            Lb = sel4(has_face, Lb, zero4());
+            Rb = sel4(has_face, Rb, zero4());
+            const prim4 Rbm = up1(Rb);  // right-state cotangent of the face below this cell
+            const prim4 sb = scale4(0.5, sub4(Lb, Rbm));
+            prim4 dlb, drb;
+            cf.apply(sb, dlb, drb);
+            prim4 Wb = add4(add4(Lb, Rbm), sub4(dlb, drb));
+            Wb = sub4(Wb, dn1(dlb));
+            Wb = add4(Wb, up1(drb));
+            if (edge) {
+                // outflow: ghost positions -1, -2 are copies of column 0 and
+                // NY, NY+1 of column NY-1 (same primitives, so the
+                // cotangents add before the primitive reverse)
+                const prim4 u1 = up1(Wb), u2 = shup4(Wb, 2);
+                const prim4 d1 = dn1(Wb), d2 = shdn4(Wb, 2);
+                if (gcol == 0) Wb = add4(Wb, add4(u1, u2));
+                if (gcol == G.NY - 1) Wb = add4(Wb, add4(d1, d2));
+            }
+            double Ub[4];
+            prim2d_rev(Y, false, Wb, a.eos, Ub);
+            if (own) {
+                double* po = a.Vb + col + (long long)(i + 2) * snye;
+                po[0] = A[0] + Ub[0];
+                po[svs] = A[1] + Ub[1];
+                po[2 * (long long)svs] = A[2] + Ub[2];
+                po[3 * (long long)svs] = A[3] + Ub[3];
+            }
+        };
 
         auto load_row = [&](int e, double* u) {
             const double* p;
@@ -135,28 +211,22 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
         }
         double Un[4];
         load_row(w.r0 + 4, Un);
-        for (int i = w.r0; i < w.r1; ++i) {
+        // X-swept row i (forward recompute), advancing the window
+        auto xstep = [&](int i, double* v) {
             double Unx[4], Uc[4];
             load_row(min(i + 5, pd.nx + 3), Unx);
             ring_get(i + 2, Uc);
             ring_put(i + 4, Un);
-            const double* pg = a.G + col + (long long)(i + 2) * snye;
-            double A[4];
-            A[0] = pg[0];
-            A[1] = pg[svs];
-            A[2] = pg[2 * (long long)svs];
-            A[3] = pg[3 * (long long)svs];
-            // ---- X-swept row (forward recompute)
             prim4 W2 = prim2d(Un[0], Un[1], Un[2], Un[3], true, a.eos);
             prim4 dW2 = sub4(W2, W1);
             prim4 s1 = slope4<LIM>(dW1, dW2);
             double Fp[4];
             face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
             FL = face_plus(W1, s1);
-            const double v0 = Uc[0] - dtdx * (Fp[0] - Fm[0]);
-            const double v1 = Uc[1] - dtdx * (Fp[1] - Fm[1]);
-            const double v2 = Uc[2] - dtdx * (Fp[3] - Fm[3]);
-            const double v3 = Uc[3] - dtdx * (Fp[2] - Fm[2]);
+            v[0] = Uc[0] - dtdx * (Fp[0] - Fm[0]);
+            v[1] = Uc[1] - dtdx * (Fp[1] - Fm[1]);
+            v[2] = Uc[2] - dtdx * (Fp[3] - Fm[3]);
+            v[3] = Uc[3] - dtdx * (Fp[2] - Fm[2]);
             W1 = W2;
             dW1 = dW2;
 #pragma unroll
@@ -164,54 +234,21 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
                 Fm[k] = Fp[k];
                 Un[k] = Unx[k];
             }
-            // ---- Y-sweep reverse across lanes
+        };
+        // software pipeline: the X-sweep of row i + 1 is issued with the
+        // Y-sweep reverse of row i (the two do not depend on each other);
+        // past the strip's last row it computes a row that is discarded
+        // (its loads are clamped to the patch)
+        double vc[4];
+        xstep(w.r0, vc);
+        for (int i = w.r0; i < w.r1; ++i) {
+            double A[4];
+            load_int(a.G, i, A);
+            double vn[4];
+            xstep(i + 1, vn);
+            yrev(i, vc, A);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) A[k] = real ? a.g * A[k] : 0.0;
-            const prim4 Y = prim2d(v0, v1, v2, v3, false, a.eos);
-            const prim4 dl = sub4(Y, up1(Y));
-            const prim4 dr = dn1(dl);
-            const prim4 sy = slope4<LIM>(dl, dr);
-            const coef4 cf = limiter_coef4<LIM>(dl, dr);
-            const prim4 Lf = face_plus(Y, sy);
-            const prim4 Rf = dn1(face_minus(Y, sy));
-            // face between this lane and the next: flux cotangent from the
-            // output cotangents of the two cells (y_sweep_t update order:
-            // rho <- G0, m_u <- G3, m_v <- G1, E <- G2)
-            double An[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) An[k] = lane_down(A[k]);
-            double Fb[4];
-            Fb[0] = dtdy * (An[0] - A[0]);
-            Fb[1] = dtdy * (An[2] - A[2]);
-            Fb[2] = dtdy * (An[3] - A[3]);
-            Fb[3] = dtdy * (An[1] - A[1]);
-            prim4 Lb = zero4(), Rb = zero4();
-            if (lane >= 1 && lane <= 61) face_flux_rev(Lf, Rf, a.eos, Fb, Lb, Rb);
-            const prim4 Rbm = up1(Rb);  // right-state cotangent of the face below this cell
-            const prim4 sb = scale4(0.5, sub4(Lb, Rbm));
-            const prim4 dlb = mul4(cf.l, sb);
-            const prim4 drb = mul4(cf.r, sb);
-            prim4 Wb = add4(add4(Lb, Rbm), sub4(dlb, drb));
-            Wb = sub4(Wb, dn1(dlb));
-            Wb = add4(Wb, up1(drb));
-            if (edge) {
-                // outflow: ghost positions -1, -2 are copies of column 0 and
-                // NY, NY+1 of column NY-1 (same primitives, so the
-                // cotangents add before the primitive reverse)
-                const prim4 u1 = up1(Wb), u2 = shup4(Wb, 2);
-                const prim4 d1 = dn1(Wb), d2 = shdn4(Wb, 2);
-                if (gcol == 0) Wb = add4(Wb, add4(u1, u2));
-                if (gcol == G.NY - 1) Wb = add4(Wb, add4(d1, d2));
-            }
-            double Ub[4];
-            prim2d_rev(Y, false, Wb, a.eos, Ub);
-            if (own) {
-                double* po = a.Vb + col + (long long)(i + 2) * snye;
-                po[0] = A[0] + Ub[0];
-                po[svs] = A[1] + Ub[1];
-                po[2 * (long long)svs] = A[2] + Ub[2];
-                po[3 * (long long)svs] = A[3] + Ub[3];
-            }
+            for (int k = 0; k < 4; ++k) vc[k] = vn[k];
         }
     }
 }
@@ -224,7 +261,7 @@ __global__ __launch_bounds__(256) void adj_y_kernel(adj_args a) {
 // OUTX: some problem of the launch has outflow x boundaries (the ghost-row
 // fold is compiled in only then).
 template <int LIM, bool OUTX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void adj_x_kernel(adj_args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_WAVES, 8))) void adj_x_kernel(adj_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
@@ -293,22 +330,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
             }
         };
 
-        const int tb = w.r0 - 3 - (lo ? 2 : 0);
+        // The first face that reaches an emitted cell is r0 - 3/2 (through
+        // the slope of cell r0 - 1), reversed at t = r0: the march starts
+        // there with the window W(r0-3 .. r0-1) and zero cotangents (faces
+        // before it touch only cells below r0; at an outflow end the faces
+        // past the ghosts carry zero cotangent). Cell r1 - 1 completes at
+        // t = r1 + 2, the ghosts past an outflow end two steps later.
         const int te = w.r1 + 3 + (hi ? 2 : 0);
         double u[4];
-        load_pos(a.U, tb, u);
+        load_pos(a.U, w.r0 - 3, u);
         prim4 W3 = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
-        prim4 W2 = W3, W1 = W3;
+        load_pos(a.U, w.r0 - 2, u);
+        prim4 W2 = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
+        load_pos(a.U, w.r0 - 1, u);
+        prim4 W1 = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
         double Vb2[4] = {0.0, 0.0, 0.0, 0.0};
+        if (interior(w.r0 - 2)) load_pos(a.Vb, w.r0 - 2, Vb2);
         prim4 Wb3 = zero4(), Wb2 = zero4(), sb2 = zero4(), acc = zero4();
-        for (int t = tb + 1; t < te; ++t) {
-            load_pos(a.U, t, u);
-            const prim4 Wt = prim2d(u[0], u[1], u[2], u[3], true, a.eos);
+        prim4 s2 = slope4<LIM>(sub4(W2, W3), sub4(W1, W2));  // slope of cell t-2, carried
+        // one march position: face t - 3/2 reversed, the cotangent of cell
+        // t - 3 completed in Wb3
+        auto body = [&](int t) {
+            double uu[4];
+            load_pos(a.U, t, uu);
+            const prim4 Wt = prim2d(uu[0], uu[1], uu[2], uu[3], true, a.eos);
             double Vb1[4] = {0.0, 0.0, 0.0, 0.0};
-            if (interior(t - 1)) load_pos(a.Vb, t - 1, Vb1);
+            if (!OUTX || interior(t - 1)) load_pos(a.Vb, t - 1, Vb1);
             const prim4 d3 = sub4(W2, W3), d2 = sub4(W1, W2), d1 = sub4(Wt, W1);
-            const prim4 s2 = slope4<LIM>(d3, d2);
             const prim4 s1 = slope4<LIM>(d2, d1);
+            const limsel4<LIM> cf = limiter_sel4<LIM>(d3, d2);
             // face t-3/2 (x_sweep_t update order: rho <- F0, m_u <- F1, m_v <- F3, E <- F2)
             double Fb[4];
             Fb[0] = dtdx * (Vb1[0] - Vb2[0]);
@@ -321,65 +371,102 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
             sb2 = add4(sb2, scale4(0.5, Lb));  // slope cotangent of cell t-2 complete
             prim4 Wb1 = Rb;
             const prim4 sb1 = scale4(-0.5, Rb);
-            const coef4 cf = limiter_coef4<LIM>(d3, d2);
-            const prim4 dlb = mul4(cf.l, sb2), drb = mul4(cf.r, sb2);
+            prim4 dlb, drb;
+            cf.apply(sb2, dlb, drb);
             Wb2 = add4(Wb2, sub4(dlb, drb));
             Wb3 = sub4(Wb3, dlb);  // primitive cotangent of cell t-3 complete
             Wb1 = add4(Wb1, drb);
-
+            return [&, Wt, Wb1, sb1, s1, Vb1]() {  // the shift to position t + 1
+                W3 = W2;
+                W2 = W1;
+                W1 = Wt;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Vb2[k] = Vb1[k];
+                Wb3 = Wb2;
+                Wb2 = Wb1;
+                sb2 = sb1;
+                s2 = s1;
+            };
+        };
+        // any position, with the outflow ends: ghost cotangents fold into
+        // the end cells (row 0 gets those of positions -2, -1; row NX-1 those
+        // of NX, NX+1, emitted once the last of them is complete)
+        auto slow = [&](int t) {
+            auto shift = body(t);
             const int p = t - 3;
-            if (lo && p < 0) {
-                acc = add4(acc, Wb3);
-            } else if (lo && p == 0) {
-                emit(0, W3, add4(Wb3, acc));
-                acc = zero4();
-            } else if (hi && p == w.r1 - 1) {
-                acc = Wb3;
-            } else if (hi && p >= w.r1) {
+            if (p < w.r0) {
+                if (lo) acc = add4(acc, Wb3);
+            } else if (p < w.r1) {
+                prim4 val = Wb3;
+                if (lo && p == 0) {
+                    val = add4(val, acc);
+                    acc = zero4();
+                }
+                if (hi && p == w.r1 - 1)
+                    acc = val;
+                else
+                    emit(p, W3, val);
+            } else {
                 acc = add4(acc, Wb3);
                 if (p == w.r1 + 1) emit(w.r1 - 1, W3, acc);  // W3: ghost copy of row NX-1
-            } else if (p >= w.r0 && p < w.r1) {
-                emit(p, W3, Wb3);
             }
-            W3 = W2;
-            W2 = W1;
-            W1 = Wt;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) Vb2[k] = Vb1[k];
-            Wb3 = Wb2;
-            Wb2 = Wb1;
-            sb2 = sb1;
+            shift();
+        };
+        // the interior of the strip: emit every position (no end cases), two
+        // positions per trip so that the two face reverses, which do not
+        // depend on each other, can be interleaved
+        auto fast = [&](int t) {
+            auto shift = body(t);
+            emit(t - 3, W3, Wb3);
+            shift();
+        };
+        const int ta = w.r0 + 3 + (lo ? 1 : 0);  // first position of the interior part
+        const int tz = w.r1 + 3 - (hi ? 1 : 0);  // its end
+        int t = w.r0;
+        for (; t < ta && t < te; ++t) slow(t);
+        for (; t + 1 < tz; t += 2) {
+            fast(t);
+            fast(t + 1);
         }
+        for (; t < tz; ++t) fast(t);
+        for (; t < te; ++t) slow(t);
     }
 }
 
 // Copies between the patch layout and the problem-major interior layout
 // [problem][var][x][y] (problem q starts at goff[q]): dir 0 patch -> global,
-// dir 1 global -> patch (interior cells only; ghosts untouched).
+// dir 1 global -> patch (interior cells only; ghosts untouched). Block
+// (patch, variable, z) copies rows [PACK_ROWS z, PACK_ROWS (z+1)) of the
+// patch, threads along y (both sides contiguous).
+constexpr int PACK_ROWS = 4;
 __global__ __launch_bounds__(256) void pack_kernel(const double* src, double* dst, const nest_patch_desc* P,
                                                    const nest_prob_geom* geoms, const long long* goff, int dir) {
     const nest_patch_desc d = P[blockIdx.x];
     const int k = blockIdx.y;
+    const int x0 = blockIdx.z * PACK_ROWS;
+    if (x0 >= d.nx) return;
     const nest_prob_geom G = geoms[d.prob];
     const int nye = d.ny + 4;
     const long long vs = (long long)(d.nx + 4) * nye;
     const long long gbase = goff[d.prob] + (long long)k * G.NX * G.NY;
-    const int n = d.nx * d.ny;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const int ix = c / d.ny, iy = c - ix * d.ny;
-        const long long ps = d.off + k * vs + (long long)(ix + 2) * nye + iy + 2;
-        const long long gi = gbase + (long long)(d.x0 + ix) * G.NY + d.y0 + iy;
-        if (dir == 0)
-            dst[gi] = src[ps];
-        else
-            dst[ps] = src[gi];
+    const int x1 = min(x0 + PACK_ROWS, d.nx);
+    for (int ix = x0; ix < x1; ++ix) {
+        const long long ps = d.off + k * vs + (long long)(ix + 2) * nye + 2;
+        const long long gi = gbase + (long long)(d.x0 + ix) * G.NY + d.y0;
+        for (int iy = threadIdx.x; iy < d.ny; iy += blockDim.x) {
+            if (dir == 0)
+                dst[gi + iy] = src[ps + iy];
+            else
+                dst[ps + iy] = src[gi + iy];
+        }
     }
 }
 
-void launch_pack(const double* src, double* dst, const nest_patch_desc* P, int npatch, const nest_prob_geom* geoms,
-                 const long long* goff, int dir, hipStream_t s) {
-    if (npatch <= 0) return;
-    hipLaunchKernelGGL(pack_kernel, dim3(npatch, 4), dim3(256), 0, s, src, dst, P, geoms, goff, dir);
+void launch_pack(const double* src, double* dst, const nest_patch_desc* P, int npatch, int max_nx,
+                 const nest_prob_geom* geoms, const long long* goff, int dir, hipStream_t s) {
+    if (npatch <= 0 || max_nx <= 0) return;
+    const int nz = (max_nx + PACK_ROWS - 1) / PACK_ROWS;
+    hipLaunchKernelGGL(pack_kernel, dim3(npatch, 4, nz), dim3(256), 0, s, src, dst, P, geoms, goff, dir);
 }
 
 void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s) {

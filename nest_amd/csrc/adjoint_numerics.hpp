@@ -55,6 +55,46 @@ __device__ __forceinline__ coef4 limiter_coef4(const prim4& dl, const prim4& dr)
     limiter_coef<LIM>(dl.q, dr.q, c.l.q, c.r.q);
     return c;
 }
+// The limiter's branch of all 4 quantities, decided from (dl, dr) and applied
+// later to the slope cotangent sb: dlb = cl*sb, drb = cr*sb. minmod keeps
+// only which difference it picked (two lane masks per quantity, no
+// floating-point coefficients); the other limiters keep the coefficients.
+template <int LIM>
+struct limsel4 {
+    coef4 c;
+    __device__ __forceinline__ void apply(const prim4& sb, prim4& dlb, prim4& drb) const {
+        dlb = {c.l.r * sb.r, c.l.u * sb.u, c.l.p * sb.p, c.l.q * sb.q};
+        drb = {c.r.r * sb.r, c.r.u * sb.u, c.r.p * sb.p, c.r.q * sb.q};
+    }
+};
+template <>
+struct limsel4<LIM_MINMOD> {
+    bool l[4], r[4];
+    __device__ __forceinline__ void apply(const prim4& sb, prim4& dlb, prim4& drb) const {
+        dlb = {l[0] ? sb.r : 0.0, l[1] ? sb.u : 0.0, l[2] ? sb.p : 0.0, l[3] ? sb.q : 0.0};
+        drb = {r[0] ? sb.r : 0.0, r[1] ? sb.u : 0.0, r[2] ? sb.p : 0.0, r[3] ? sb.q : 0.0};
+    }
+};
+__device__ __forceinline__ void minmod_pick(double a, double b, bool& pa, bool& pb) {
+    const bool live = !(a * b <= 0.0);
+    const bool pick_a = __builtin_fabs(a) < __builtin_fabs(b);
+    pa = live && pick_a;
+    pb = live && !pick_a;
+}
+template <int LIM>
+__device__ __forceinline__ limsel4<LIM> limiter_sel4(const prim4& dl, const prim4& dr) {
+    limsel4<LIM> s;
+    if constexpr (LIM == LIM_MINMOD) {
+        minmod_pick(dl.r, dr.r, s.l[0], s.r[0]);
+        minmod_pick(dl.u, dr.u, s.l[1], s.r[1]);
+        minmod_pick(dl.p, dr.p, s.l[2], s.r[2]);
+        minmod_pick(dl.q, dr.q, s.l[3], s.r[3]);
+    } else {
+        s.c = limiter_coef4<LIM>(dl, dr);
+    }
+    return s;
+}
+
 __device__ __forceinline__ prim4 mul4(const prim4& a, const prim4& b) {
     return {a.r * b.r, a.u * b.u, a.p * b.p, a.q * b.q};
 }
@@ -92,12 +132,25 @@ __device__ __forceinline__ void prim2d_rev(const prim4& W, bool xsweep, const pr
 // face flux reverse: face_flux<true> (HLLE with Davis speeds on (rho, u_n, p)
 // plus the passive transverse-momentum flux). Fb = cotangent of F[0..3];
 // accumulates the cotangents of the two face states into Lb, Rb.
+// Branch-free: the subsonic (HLLE-average) terms are formed for every face
+// (SR > SL always, since c > 0) and the branch the primal takes selects
+// which terms are passed on, so a wave has no divergent paths here.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void face_flux_rev(const prim4& L, const prim4& R, const eos_c& e, const double Fb[4],
                                               prim4& Lb, prim4& Rb) {
     // ---- primal recompute, as face_flux<true>
+#if NEST_FAST
+    // c = (gamma p) y with y = rsq(gamma p rho), as sound_speed; y also gives
+    // the derivatives without a division: dc/dp = gamma y / 2, dc/drho =
+    // -c / (2 rho) = -c^2 y / 2
+    const double gpL = e.gamma * L.p, gpR = e.gamma * R.p;
+    const double yL = rsq(gpL * L.r), yR = rsq(gpR * R.r);
+    const double cL = gpL * yL;
+    const double cR = gpR * yR;
+#else
     const double cL = sound_speed(L.r, L.p, e);
     const double cR = sound_speed(R.r, R.p, e);
+#endif
     const double EL = total_energy(L.r, L.u, L.p, e);
     const double ER = total_energy(R.r, R.u, R.p, e);
     const double aL = L.u - cL, aR = R.u - cR;
@@ -108,63 +161,51 @@ __device__ __forceinline__ void face_flux_rev(const prim4& L, const prim4& R, co
     const double SR = sr_from_r ? bR : bL;
     const bool left = SL >= 0.0;
     const bool right = !left && (SR <= 0.0);
+    const bool mid = !left && !right;
     const double FL[3] = {L.r * L.u, L.r * L.u * L.u + L.p, (EL + L.p) * L.u};
     const double FR[3] = {R.r * R.u, R.r * R.u * R.u + R.p, (ER + R.p) * R.u};
-
-    double Fbl[3] = {0.0, 0.0, 0.0}, Fbr[3] = {0.0, 0.0, 0.0};
-    double ELb = 0.0, ERb = 0.0, SLb = 0.0, SRb = 0.0;
-    double ui, uib;
-    if (left) {
-        ui = L.u;
-    } else if (right) {
-        ui = R.u;
-    } else {
-        ui = ddiv(SR * L.u - SL * R.u, SR - SL);
-    }
+#if NEST_FAST
+    const double inv = rcp(SR - SL);
+    const double um = (SR * L.u - SL * R.u) * inv;
+#else
+    const double inv = 1.0 / (SR - SL);
+    const double um = (SR * L.u - SL * R.u) / (SR - SL);
+#endif
+    const double ui = left ? L.u : (right ? R.u : um);
     // passive flux F3 = ui * (ui >= 0 ? qL : qR)  (euler2d.hpp:27-33)
-    if (ui >= 0.0) {
-        uib = Fb[3] * L.q;
-        Lb.q += Fb[3] * ui;
-    } else {
-        uib = Fb[3] * R.q;
-        Rb.q += Fb[3] * ui;
-    }
-    if (left) {
-        Fbl[0] = Fb[0], Fbl[1] = Fb[1], Fbl[2] = Fb[2];
-        Lb.u += uib;
-    } else if (right) {
-        Fbr[0] = Fb[0], Fbr[1] = Fb[1], Fbr[2] = Fb[2];
-        Rb.u += uib;
-    } else {
-        // H_k = (SR FL_k - SL FR_k + SL SR (UR_k - UL_k)) / (SR - SL), k = 0..2,
-        // U = (r, r u, E); interface velocity (SR uL - SL uR) / (SR - SL)
-        const double inv = ddiv(1.0, SR - SL);
-        const double dU[3] = {R.r - L.r, R.r * R.u - L.r * L.u, ER - EL};
-        const double sLR = SL * SR;
-        double sr_acc = 0.0, sl_acc = 0.0;
+    const bool upw = ui >= 0.0;
+    const double uib = Fb[3] * (upw ? L.q : R.q);
+    Lb.q += upw ? Fb[3] * ui : 0.0;
+    Rb.q += upw ? 0.0 : Fb[3] * ui;
+
+    // subsonic branch: H_k = (SR FL_k - SL FR_k + SL SR (UR_k - UL_k)) / (SR - SL),
+    // k = 0..2, U = (r, r u, E); interface velocity (SR uL - SL uR) / (SR - SL)
+    const double dU[3] = {R.r - L.r, R.r * R.u - L.r * L.u, ER - EL};
+    const double sLR = SL * SR;
+    double sr_acc = 0.0, sl_acc = 0.0;
+    double Fbl[3], Fbr[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double H = (SR * FL[k] - SL * FR[k] + sLR * dU[k]) * inv;
-            sr_acc += Fb[k] * (FL[k] + SL * dU[k] - H);
-            sl_acc += Fb[k] * (SR * dU[k] - FR[k] + H);
-            Fbl[k] = Fb[k] * SR * inv;
-            Fbr[k] = -Fb[k] * SL * inv;
-        }
-        sr_acc += uib * (L.u - ui);
-        sl_acc += uib * (ui - R.u);
-        SRb = sr_acc * inv;
-        SLb = sl_acc * inv;
-        Lb.u += uib * SR * inv;
-        Rb.u -= uib * SL * inv;
-        // state jump (UR_k - UL_k) weighted by SL SR / (SR - SL)
-        const double j0 = Fb[0] * sLR * inv, j1 = Fb[1] * sLR * inv, j2 = Fb[2] * sLR * inv;
-        Rb.r += j0 + j1 * R.u;
-        Rb.u += j1 * R.r;
-        ERb += j2;
-        Lb.r -= j0 + j1 * L.u;
-        Lb.u -= j1 * L.r;
-        ELb -= j2;
+    for (int k = 0; k < 3; ++k) {
+        const double H = (SR * FL[k] - SL * FR[k] + sLR * dU[k]) * inv;
+        sr_acc += Fb[k] * (FL[k] + SL * dU[k] - H);
+        sl_acc += Fb[k] * (SR * dU[k] - FR[k] + H);
+        Fbl[k] = left ? Fb[k] : (mid ? Fb[k] * SR * inv : 0.0);
+        Fbr[k] = right ? Fb[k] : (mid ? -Fb[k] * SL * inv : 0.0);
     }
+    sr_acc += uib * (L.u - ui);
+    sl_acc += uib * (ui - R.u);
+    const double SRb = mid ? sr_acc * inv : 0.0;
+    const double SLb = mid ? sl_acc * inv : 0.0;
+    Lb.u += left ? uib : (mid ? uib * SR * inv : 0.0);
+    Rb.u += right ? uib : (mid ? -uib * SL * inv : 0.0);
+    // state jump (UR_k - UL_k) weighted by SL SR / (SR - SL)
+    const double w = mid ? sLR * inv : 0.0;
+    const double j0 = Fb[0] * w, j1 = Fb[1] * w, j2 = Fb[2] * w;
+    Rb.r += j0 + j1 * R.u;
+    Rb.u += j1 * R.r;
+    Lb.r -= j0 + j1 * L.u;
+    Lb.u -= j1 * L.r;
+    double ERb = j2, ELb = -j2;
     // physical fluxes F(W) = (r u, r u^2 + p, (E + p) u)
     ELb += Fbl[2] * L.u;
     Lb.r += Fbl[0] * L.u + Fbl[1] * L.u * L.u;
@@ -175,7 +216,11 @@ __device__ __forceinline__ void face_flux_rev(const prim4& L, const prim4& R, co
     Rb.u += Fbr[0] * R.r + 2.0 * Fbr[1] * R.r * R.u + Fbr[2] * (ER + R.p);
     Rb.p += Fbr[1] + Fbr[2] * R.u;
     // total energy E = p/(g-1) + r u^2 / 2  (euler1d.hpp:47-51)
-    const double igm1 = ddiv(1.0, e.gm1);
+#if NEST_FAST
+    const double igm1 = e.inv_gm1;
+#else
+    const double igm1 = 1.0 / e.gm1;
+#endif
     Lb.p += ELb * igm1;
     Lb.r += ELb * 0.5 * L.u * L.u;
     Lb.u += ELb * L.r * L.u;
@@ -183,26 +228,22 @@ __device__ __forceinline__ void face_flux_rev(const prim4& L, const prim4& R, co
     Rb.r += ERb * 0.5 * R.u * R.u;
     Rb.u += ERb * R.r * R.u;
     // Davis speeds: SL = min(uL - cL, uR - cR), SR = max(uL + cL, uR + cR)
-    double cLb = 0.0, cRb = 0.0;
-    if (sl_from_r) {
-        Rb.u += SLb;
-        cRb -= SLb;
-    } else {
-        Lb.u += SLb;
-        cLb -= SLb;
-    }
-    if (sr_from_r) {
-        Rb.u += SRb;
-        cRb += SRb;
-    } else {
-        Lb.u += SRb;
-        cLb += SRb;
-    }
+    Lb.u += (sl_from_r ? 0.0 : SLb) + (sr_from_r ? 0.0 : SRb);
+    Rb.u += (sl_from_r ? SLb : 0.0) + (sr_from_r ? SRb : 0.0);
+    const double cLb = (sr_from_r ? 0.0 : SRb) - (sl_from_r ? 0.0 : SLb);
+    const double cRb = (sr_from_r ? SRb : 0.0) - (sl_from_r ? SLb : 0.0);
     // c = sqrt(gamma p / r): dc/dp = c / (2p), dc/dr = -c / (2r)  (p > 0 in 2D)
+#if NEST_FAST
+    Lb.p += cLb * (0.5 * e.gamma * yL);
+    Lb.r -= cLb * (0.5 * cL * cL * yL);
+    Rb.p += cRb * (0.5 * e.gamma * yR);
+    Rb.r -= cRb * (0.5 * cR * cR * yR);
+#else
     Lb.p += cLb * 0.5 * ddiv(cL, L.p);
     Lb.r -= cLb * 0.5 * ddiv(cL, L.r);
     Rb.p += cRb * 0.5 * ddiv(cR, R.p);
     Rb.r -= cRb * 0.5 * ddiv(cR, R.r);
+#endif
 }
 
 }  // namespace NEST_NS

@@ -20,7 +20,8 @@
     void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s);              \
     void launch_adj_x(int lim, const adj_args& a, int grid, int block, hipStream_t s);              \
     void launch_pack(const double* src, double* dst, const nest_patch_desc* P, int npatch,          \
-                     const nest_prob_geom* geoms, const long long* goff, int dir, hipStream_t s);   \
+                     int max_nx, const nest_prob_geom* geoms, const long long* goff, int dir,        \
+                     hipStream_t s);                                                                 \
     }
 
 // arguments of the 2D stage / speed kernels

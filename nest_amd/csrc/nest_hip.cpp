@@ -18,6 +18,13 @@
 #include "nest_hip.h"
 #include "plan.hpp"
 
+// Buffer sets of the rollout adjoint's recomputed intervals: the recompute
+// of an interval (auxiliary stream) runs up to NEST_ADJ_SETS - 1 intervals
+// ahead of the reverse sweep.
+#ifndef NEST_ADJ_SETS
+#define NEST_ADJ_SETS 2
+#endif
+
 namespace {
 
 thread_local std::string g_err;
@@ -45,8 +52,8 @@ struct launch_table {
     void (*fill1d)(double*, int, int, const nest_prob_state*, hipStream_t);
     void (*adj_y)(int, const adj_args&, int, int, hipStream_t);
     void (*adj_x)(int, const adj_args&, int, int, hipStream_t);
-    void (*pack)(const double*, double*, const nest_patch_desc*, int, const nest_prob_geom*, const long long*, int,
-                 hipStream_t);
+    void (*pack)(const double*, double*, const nest_patch_desc*, int, int, const nest_prob_geom*, const long long*,
+                 int, hipStream_t);
 };
 
 const launch_table kExact = {nest_exact::launch_stage2d,     nest_exact::launch_speeds2d,
@@ -160,11 +167,12 @@ struct nest_sim2d {
     long long* goff_d = nullptr;
     std::vector<double*> ckpt;  // rollout checkpoints (every roll_m steps)
     // recomputed states of a checkpoint interval and their stage-1 states,
-    // two sets: the recompute of one interval (auxiliary stream) overlaps the
-    // reverse sweep of the next (handle's stream)
-    std::vector<double*> seg[2], seg1[2];
+    // NEST_ADJ_SETS sets: the recompute of one interval (auxiliary stream)
+    // overlaps the reverse sweep of the next (handle's stream)
+    std::vector<double*> seg[NEST_ADJ_SETS], seg1[NEST_ADJ_SETS];
     hipStream_t aux_stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
+    hipEvent_t ev_start = nullptr;
+    std::vector<hipEvent_t> ev_rec_c, ev_rev_c;  // per interval: recompute done, reverse done
     int roll_K = 0, roll_m = 0;
     double roll_dt = 0.0;
 
@@ -183,12 +191,19 @@ struct nest_sim2d {
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
                         (void*)goff_d})
             if (p) (void)hipFree(p);
-        for (auto* v : {&ckpt, &seg[0], &seg[1], &seg1[0], &seg1[1]}) {
-            for (double* p : *v) (void)hipFree(p);
+        for (double* p : ckpt) (void)hipFree(p);
+        ckpt.clear();
+        for (int k = 0; k < NEST_ADJ_SETS; ++k)
+            for (auto* v : {&seg[k], &seg1[k]}) {
+                for (double* p : *v) (void)hipFree(p);
+                v->clear();
+            }
+        if (ev_start) (void)hipEventDestroy(ev_start);
+        ev_start = nullptr;
+        for (auto* v : {&ev_rec_c, &ev_rev_c}) {
+            for (hipEvent_t e : *v) (void)hipEventDestroy(e);
             v->clear();
         }
-        for (hipEvent_t* e : {&ev_start, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
-            if (*e) (void)hipEventDestroy(*e), *e = nullptr;
         if (aux_stream) (void)hipStreamDestroy(aux_stream);
         aux_stream = nullptr;
         aitems_d[0] = aitems_d[1] = nullptr;
@@ -367,8 +382,16 @@ struct nest_sim2d {
     }
     int ensure_aux_stream() {
         if (!aux_stream) NEST_HIP_TRY(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&ev_start, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
-            if (!*e) NEST_HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        if (!ev_start) NEST_HIP_TRY(hipEventCreateWithFlags(&ev_start, hipEventDisableTiming));
+        return NEST_OK;
+    }
+    int ensure_interval_events(int n) {
+        for (auto* v : {&ev_rec_c, &ev_rev_c})
+            while ((int)v->size() < n) {
+                hipEvent_t e = nullptr;
+                NEST_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                v->push_back(e);
+            }
         return NEST_OK;
     }
     int ensure_adjoint_scratch() {
@@ -834,7 +857,10 @@ int nest_hip_sim2d_pack(nest_sim2d* s, const double* src, double* dst, int direc
     if (int rc = valid_sim(s)) return rc;
     if (!src || !dst) return fail(NEST_ERR_INVALID, "null buffer");
     if (direction != 0 && direction != 1) return fail(NEST_ERR_INVALID, "direction must be 0 or 1");
-    s->L().pack(src, dst, s->patches_d, (int)s->patches_h.size(), s->geoms_d, s->goff_d, direction, s->stream);
+    int max_nx = 0;
+    for (const auto& d : s->patches_h) max_nx = std::max(max_nx, d.nx);
+    s->L().pack(src, dst, s->patches_d, (int)s->patches_h.size(), max_nx, s->geoms_d, s->goff_d, direction,
+                s->stream);
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -879,12 +905,15 @@ int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
     const int m = interval > 0 ? interval : std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
     const int nck = (nsteps + m - 1) / m;
     if (int rc = s->grow(s->ckpt, (size_t)nck)) return rc;
+    // checkpoint c = state of step c*m: the initial state is copied; every
+    // later one is written by the stage 2 before it (out of place) and read
+    // from there by the next step, whose stage 2 writes U again
+    NEST_HIP_TRY(hipMemcpyAsync(s->ckpt[0], s->U, s->total * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     for (int k = 0; k < nsteps; ++k) {
-        if (k % m == 0)
-            NEST_HIP_TRY(hipMemcpyAsync(s->ckpt[k / m], s->U, s->total * sizeof(double), hipMemcpyDeviceToDevice,
-                                        s->stream));
-        s->stage_on(1, s->U, s->U1, dt);
-        s->stage_on(2, s->U, s->U1, dt);
+        double* src = (k % m == 0 && k > 0) ? s->ckpt[k / m] : s->U;
+        double* dst = ((k + 1) % m == 0 && k + 1 < nsteps) ? s->ckpt[(k + 1) / m] : s->U;
+        s->stage_on(1, src, s->U1, dt);
+        s->stage_on(2, src, s->U1, dt, dst == src ? nullptr : dst);
     }
     s->fill(s->U);
     s->roll_K = nsteps;
@@ -903,35 +932,45 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     const double dt = s->roll_dt;
     const int nck = (K + m - 1) / m;
     if (int rc = s->ensure_aux_stream()) return rc;
-    for (int set = 0; set < (nck > 1 ? 2 : 1); ++set) {
+    // S buffer sets: the recompute runs up to S - 1 intervals ahead of the
+    // reverse sweep
+    const int nsets = std::min(NEST_ADJ_SETS, nck);
+    for (int set = 0; set < nsets; ++set) {
         if (int rc = s->grow(s->seg[set], (size_t)std::max(0, m - 1))) return rc;
         if (int rc = s->grow(s->seg1[set], (size_t)m)) return rc;
     }
+    // one event per interval and direction (an event is never recorded
+    // again while a wait on its previous record may still be pending)
+    if (int rc = s->ensure_interval_events(nck)) return rc;
     // states of steps c*m .. c*m + len - 1 of interval c: the checkpoint, then
-    // recomputed into buffer set c % 2, with the stage-1 state of each
-    auto state = [&](int c, int i) { return i == 0 ? s->ckpt[c] : s->seg[c & 1][i - 1]; };
-    auto recompute = [&](int c) {
-        const int len = std::min(m, K - c * m), set = c & 1;
+    // recomputed into buffer set c % S, with the stage-1 state of each
+    auto state = [&](int c, int i) { return i == 0 ? s->ckpt[c] : s->seg[c % nsets][i - 1]; };
+    auto recompute = [&](int c) -> hipError_t {
+        // interval c reuses the set that the reverse of interval c + S read
+        if (c + nsets <= nck - 1) {
+            hipError_t e = hipStreamWaitEvent(s->aux_stream, s->ev_rev_c[c + nsets], 0);
+            if (e != hipSuccess) return e;
+        }
+        const int len = std::min(m, K - c * m), set = c % nsets;
         for (int i = 0; i < len; ++i) {
             s->stage_on(1, state(c, i), s->seg1[set][i], dt, nullptr, s->aux_stream);
             if (i + 1 < len) s->stage_on(2, state(c, i), s->seg1[set][i], dt, state(c, i + 1), s->aux_stream);
         }
-        return hipEventRecord(s->ev_rec[set], s->aux_stream);
+        return hipEventRecord(s->ev_rec_c[c], s->aux_stream);
     };
     // everything enqueued before (the rollout and its checkpoints) comes first
     NEST_HIP_TRY(hipEventRecord(s->ev_start, s->stream));
     NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_start, 0));
-    NEST_HIP_TRY(recompute(nck - 1));
+    for (int c = nck - 1; c >= std::max(0, nck - (nsets - 1)); --c) NEST_HIP_TRY(recompute(c));
+    // reverse of interval c (handle's stream), then the recompute S - 1
+    // intervals further down (auxiliary stream; it waits for the reverse
+    // that last read its buffer set)
     for (int c = nck - 1; c >= 0; --c) {
-        const int set = c & 1, len = std::min(m, K - c * m);
-        NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec[set], 0));
-        if (c >= 1) {
-            // interval c-1 reuses the set that the reverse of interval c+1 read
-            if (c + 1 <= nck - 1) NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_rev[(c - 1) & 1], 0));
-            NEST_HIP_TRY(recompute(c - 1));
-        }
+        const int set = c % nsets, len = std::min(m, K - c * m);
+        NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
         for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
-        NEST_HIP_TRY(hipEventRecord(s->ev_rev[set], s->stream));
+        NEST_HIP_TRY(hipEventRecord(s->ev_rev_c[c], s->stream));
+        if (c - (nsets - 1) >= 0) NEST_HIP_TRY(recompute(c - (nsets - 1)));
     }
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
