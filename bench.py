@@ -67,25 +67,41 @@ def cpu_baseline(G, budget_s: float):
                       f"of {os.cpu_count()} visible ({cpu_model()})"}
 
 
-def copy_peak_gbs(gib: int = 2, reps: int = 5) -> float:
-    """Measured device-copy bandwidth: median of `reps` torch D2D copies of
-    `gib` GiB (read + write bytes / time, HIP events)."""
+def copy_peak_gbs(gib: int = 2, reps: int = 10) -> dict:
+    """Measured device-copy bandwidth (read + write bytes / time, HIP events
+    on the launching stream), best of `reps` copies of `gib` GiB: the
+    library's plain copy kernel (nest_hip_device_copy) and torch's copy."""
+    import ctypes as C
+
     import torch
 
+    from nest_amd import _lib
+
+    L = _lib.load()
     n = gib * (1 << 30) // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
-    ts = []
-    for _ in range(reps):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        dst.copy_(src)
-        b.record()
-        b.synchronize()
-        ts.append(a.elapsed_time(b) * 1e-3)
+    stream = torch.cuda.current_stream()
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ts.append(a.elapsed_time(b) * 1e-3)
+        return 2.0 * n * 8 / min(ts) / 1e9
+
+    def lib_copy():
+        _lib.check(L.nest_hip_device_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), n,
+                                          C.c_void_p(stream.cuda_stream)))
+
+    out = {"library_copy_kernel": round(best(lib_copy), 1), "torch_copy": round(best(lambda: dst.copy_(src)), 1)}
     del src, dst
-    return 2.0 * n * 8 / sorted(ts)[reps // 2] / 1e9
+    return out
 
 
 def cpu_model() -> str:
@@ -345,9 +361,16 @@ def main():
     log(f"bench: building 4096^2 Sod state ({args.mode})")
     G = P.sod2d(N, N, jitter=1e-3, seed=2)
     sim = Sim2D.mesh(N, N, NP, NP, mode=args.mode)
+    U = sim.scatter(G)
+    tiling = None
     if args.rows or args.wpb:
         sim.set_tiling(args.rows, args.wpb, 0)
-    U = sim.scatter(G)
+    else:
+        from nest_amd import autotune
+        sim.upload(U)
+        sim.fill_halos()
+        sim.start(cfl=0.4)
+        tiling = autotune.tune(sim, [(N, N)], log=log)
     sim.upload(U)
     sim.fill_halos()
     sim.start(cfl=0.4)
@@ -431,17 +454,26 @@ def main():
         cpu = cpu_baseline(G, args.cpu_budget)
 
     if args.copy_probe_mib > 0:
-        # a plain device copy of known size, for scaling the FETCH_SIZE /
-        # WRITE_SIZE counters of profiled runs (not part of any timed region)
+        # a plain device copy of known size (the library's copy kernel), for
+        # scaling the FETCH_SIZE / WRITE_SIZE counters of profiled runs (not
+        # part of any timed region)
+        import ctypes as C
+
         import torch
-        src = torch.ones(args.copy_probe_mib * 131072, dtype=torch.float64, device="cuda")
+
+        from nest_amd import _lib
+        n = args.copy_probe_mib * 131072
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        _lib.check(_lib.load().nest_hip_device_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), n,
+                                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         torch.cuda.synchronize()
         del src, dst
 
-    copy_gbs = copy_peak_gbs()
+    copies = copy_peak_gbs()
+    copy_gbs = copies["library_copy_kernel"]
     traffic = load_traffic()
+    tr_bytes = (traffic or {}).get("stage2_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -463,6 +495,7 @@ def main():
         "stages_hbm_frac": round(stages_gbs / HBM_PEAK_GBS, 4),
         "kernel_ms": {"stage1": round(st1_ms, 4), "stage2": round(st2_ms, 4),
                       "finalize": round(comp_ms[0] / args.steps, 4)},
+        "tiling": tiling,
         "graph_value": round(graph_value, 1),
         "per_patch_value": pp,
         "ensemble": ens,
@@ -472,7 +505,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),
                      "measured_copy_peak": round(copy_gbs, 1), "frac_vs_copy_peak": round(achieved / copy_gbs, 4),
-                     "traffic": (traffic or {}).get("stage2_bytes_per_launch")},
+                     "copy_kernels": copies,
+                     "traffic": tr_bytes,
+                     # measured bytes (rocprof counters) moved per second of the launch
+                     "traffic_gbs": round(tr_bytes / (st2_ms * 1e-3) / 1e9, 1) if tr_bytes else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

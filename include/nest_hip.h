@@ -51,6 +51,11 @@ const char* nest_hip_last_error(void);
 int nest_hip_version(void);
 /* Number of visible HIP devices (0 if none); does not create a context. */
 int nest_hip_device_count(void);
+/* dst[i] = src[i], i < n, on `stream` (null: the null stream): a plain
+   device copy, 8 B per lane, grid-stride. No counterpart in the reference:
+   it is the measured HBM ceiling the stage kernels are compared with, and
+   the known-size kernel that calibrates the memory-traffic counters. */
+int nest_hip_device_copy(const double* src, double* dst, long long n, void* stream);
 
 /* ===================================================================== */
 /* 1D single-patch solver. Replaces the CUDA solver object of reference   */

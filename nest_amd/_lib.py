@@ -40,6 +40,7 @@ SIGNATURES = {
     "nest_hip_last_error": (C.c_char_p, []),
     "nest_hip_version": (_i, []),
     "nest_hip_device_count": (_i, []),
+    "nest_hip_device_copy": (_i, [_vp, _vp, _ll, _vp]),
     "nest_hip_solver1d_create": (_i, [_i, _d, _d, _i, _i, _i, C.POINTER(_vp)]),
     "nest_hip_solver1d_destroy": (_i, [_vp]),
     "nest_hip_solver1d_upload": (_i, [_vp, _pd]),

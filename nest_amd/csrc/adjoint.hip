@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         u[2] = q[128];
         u[3] = q[192];
     };
-    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
         const nest_prob_state& ps = a.probs[pd.prob];
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
-    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
         const nest_prob_state& ps = a.probs[pd.prob];

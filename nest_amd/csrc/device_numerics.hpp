@@ -223,6 +223,26 @@ __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const 
 }
 
 #if defined(__HIPCC__)
+// XCD-aware block order. The dispatcher hands block b of a launch to XCD
+// b % 8 (gfx950: 8 XCDs, each with its own L2). Renumbering the blocks so
+// that each XCD runs one contiguous run of the work list puts neighbouring
+// strips (which share halo columns and partly filled cache lines) and the
+// strip below each one (whose first rows are the last rows of this one) in
+// the same L2. A bijection on [0, nb).
+#ifndef NEST_XCD_SWIZZLE
+#define NEST_XCD_SWIZZLE 1
+#endif
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+#if NEST_XCD_SWIZZLE
+    constexpr int X = 8;
+    const int q = nb / X, r = nb % X, x = b % X;
+    return x * q + (x < r ? x : r) + b / X;
+#else
+    (void)nb;
+    return b;
+#endif
+}
+
 // Neighbour-lane values by DPP wave shifts (a VALU move, no LDS round trip):
 // lane_up(x) on lane l is x of lane l-1 and lane_down(x) is x of lane l+1;
 // lanes 0 / 63 keep their own value (the semantics of __shfl_up / __shfl_down

@@ -36,6 +36,12 @@
 #ifndef NEST_STAGE1_UNROLL
 #define NEST_STAGE1_UNROLL 1
 #endif
+// Software pipeline of the march (1): the X step of row i + 1 is issued
+// together with the Y step of row i (they do not depend on each other), and
+// the Y step has no branch before its stores.
+#ifndef NEST_STAGE_PIPE
+#define NEST_STAGE_PIPE 0
+#endif
 #define NEST_STAGE_OCCUPANCY \
     __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? NEST_STAGE1_WAVES : NEST_STAGE2_WAVES, 8)))
 
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
         u[2] = q[128];
         u[3] = q[192];
     };
-    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
         nest_prob_state* ps = a.probs + pd.prob;
@@ -227,7 +233,9 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             double* po = out + (size_t)(i + 2) * snye;
             double U0[4];  // U^n at the output cell (stage 2 combine), issued early
             if constexpr (STAGE == 2) {
-                if (store) {
+                // (every lane's po is an in-bounds cell: lanes that store
+                // nothing load a value they do not use)
+                if (NEST_STAGE_PIPE || store) {
                     const double* pu = po + u0_off;
                     U0[0] = pu[0];
                     U0[1] = pu[vs];
@@ -251,7 +259,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             double o3 = ux[3] - dtdy * (G[2] - Gm[2]);
             if constexpr (STAGE == 2) {
                 // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2)
-                if (store) {
+                if (NEST_STAGE_PIPE || store) {
                     o0 = 0.5 * (U0[0] + o0);
                     o1 = 0.5 * (U0[1] + o1);
                     o2 = 0.5 * (U0[2] + o2);
@@ -260,8 +268,8 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                     cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
                     unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
                     unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
-                    smx = bx > smx ? bx : smx;
-                    smy = by > smy ? by : smy;
+                    smx = (store && bx > smx) ? bx : smx;
+                    smy = (store && by > smy) ? by : smy;
                 }
             }
             if (store) {
@@ -271,7 +279,38 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                 po[3 * (size_t)vs] = o3;
             }
         };
-        if constexpr (STAGE == 1 && NEST_STAGE1_UNROLL) {
+        if constexpr (NEST_STAGE_PIPE && STAGE == 1) {
+            // two row buffers alternate (as below); X step of row i + 1 with
+            // the Y step of row i
+            double Vc[4], Vn[4];
+            xstep(w.r0, Vc, UA, 2);
+            int i = w.r0;
+            for (; i + 2 < w.r1; i += 2) {
+                xstep(i + 1, Vn, UB, 2);
+                ystep(i, Vc);
+                xstep(i + 2, Vc, UA, 2);
+                ystep(i + 1, Vn);
+            }
+            if (i + 1 < w.r1) {
+                xstep(i + 1, Vn, UB, 2);
+                ystep(i, Vc);
+                ystep(i + 1, Vn);
+            } else {
+                ystep(i, Vc);
+            }
+        } else if constexpr (NEST_STAGE_PIPE) {
+            double Vc[4];
+            xstep(w.r0, Vc, UA, 1);
+            int i = w.r0;
+            for (; i + 1 < w.r1; ++i) {
+                double Vn[4];
+                xstep(i + 1, Vn, UA, 1);
+                ystep(i, Vc);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Vc[k] = Vn[k];
+            }
+            ystep(i, Vc);
+        } else if constexpr (STAGE == 1 && NEST_STAGE1_UNROLL) {
             // the march, unrolled by two so that the two row buffers
             // alternate without register copies (a copy would wait for the
             // load at once): each row load has two steps to arrive
@@ -309,7 +348,7 @@ __global__ __launch_bounds__(256) void speeds2d_kernel(stage_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
-    for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
         nest_prob_state* ps = a.probs + pd.prob;

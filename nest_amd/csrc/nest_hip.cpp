@@ -481,6 +481,27 @@ int nest_hip_device_count(void) {
     return n;
 }
 
+}  // extern "C"
+
+namespace {
+__global__ __launch_bounds__(256) void device_copy_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                          long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int nest_hip_device_copy(const double* src, double* dst, long long n, void* stream) {
+    if (n < 0 || (n > 0 && (!src || !dst))) return fail(NEST_ERR_INVALID, "need n >= 0 and buffers");
+    if (n == 0) return NEST_OK;
+    const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(device_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, src, dst, n);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
 int nest_hip_mesh2d_create(int nx, int ny, int npx, int npy, double dx, double dy, double gamma, int bcx, int bcy,
                            int limiter, int mode, nest_sim2d** out) {
     if (!out) return fail(NEST_ERR_INVALID, "out is null");
