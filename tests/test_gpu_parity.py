@@ -346,3 +346,28 @@ def test_config3_ensemble_batched_equals_per_patch_and_oracle(oracle):
     # every problem: identical between the two paths
     assert np.array_equal(ga, gb) or all(
         np.array_equal(a.patch_view(ga, k)[:, 2:-2, 2:-2], b.patch_view(gb, k)[:, 2:-2, 2:-2]) for k in range(512))
+
+
+def test_device_copy_kernel():
+    """The plain copy kernel (the measured bandwidth ceiling and counter
+    calibration): exact copy of sizes around its grid cap and block size,
+    including an empty copy and a size above 65536 * 256 elements (grid
+    stride), and it rejects a negative size."""
+    import ctypes as C
+
+    import torch
+
+    from nest_amd import _lib
+    print_case("device copy")
+    L = _lib.load()
+    for n in (0, 1, 255, 257, 65536 * 256 + 3):
+        src = torch.arange(n, dtype=torch.float64, device="cuda") * 0.5 - 7.0
+        dst = torch.full((max(n, 1),), -1.0, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(L.nest_hip_device_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), n, None))
+        torch.cuda.synchronize()
+        assert torch.equal(dst[:n], src)
+        if n == 0:
+            assert dst[0].item() == -1.0
+    with pytest.raises(_lib.NestError):
+        _lib.check(L.nest_hip_device_copy(None, None, -1, None))
