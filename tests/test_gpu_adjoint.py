@@ -106,6 +106,33 @@ def test_stage_adjoint_full_transpose(oracle, case, lim):
     sim.close()
 
 
+@pytest.mark.parametrize("bcs", [("outflow", "outflow"), ("periodic", "outflow"), ("outflow", "periodic")])
+def test_stage_adjoint_short_strips(oracle, bcs):
+    """Adjoint work items of 1..5 rows (the X kernel's march with and
+    without an interior part, items that start or end at an outflow end, a
+    one-row item touching both its patch's ends), against the full dual
+    J^T w: every tiling gives the same vector."""
+    bcx, bcy = bcs
+    nx, ny, npx, npy = 14, 12, 2, 2
+    print_case("adjoint short strips", bcs)
+    m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+    sim = Sim2D.mesh(nx, ny, npx, npy, bcx=bcx, bcy=bcy, mode="exact")
+    dx, dy, dt = 1.0 / nx, 1.0 / ny, 0.2 / max(nx, ny) / 3.0
+    U = sim.scatter(rough_state(nx, ny, seed=41))
+    w = np.random.default_rng(42).standard_normal((4, nx, ny))
+
+    def jvp(Uc, dU):
+        oracle.mesh_stage_jvp(Uc, dU, m, dt, dx, dy)
+
+    want = _jacobian_transpose_w(oracle, U, w, m, sim, jvp)
+    for rows_y, rows_x in ((1, 1), (2, 2), (3, 3), (1, 5), (2, 4), (6, 7)):
+        sim.set_adjoint_tiling(rows_y, 1, rows_x, 2)
+        got = sim.gather(gpu_stage_vjp(sim, U, sim.scatter(w), dt))
+        err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        assert err <= TOL, (rows_y, rows_x, err)
+    sim.close()
+
+
 @pytest.mark.parametrize("case", CASES[:2])
 def test_step_adjoint_full_transpose(oracle, case):
     nx, ny, npx, npy, bcx, bcy = case
