@@ -116,7 +116,7 @@ def cpu_model() -> str:
 
 
 def ensemble_leg(steps: int, mode: str):
-    """Config 3: 512 uneven four-quadrant problems, 100 RK2 steps with
+    """Ensemble: 512 uneven four-quadrant problems, 100 RK2 steps with
     per-problem device dt. Batched (one launch per stage over all problems,
     hipGraph-replayed step, autotuned tiles) against the plain per-patch path
     (host dt with a blocking read-back, one launch set per problem)."""
@@ -152,15 +152,15 @@ def ensemble_leg(steps: int, mode: str):
 
 
 def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps: int = 3):
-    """Config 4: 1024^2 periodic smooth state, K fixed-dt RK2 steps, loss =
+    """Adjoint: 1024^2 periodic smooth state, K fixed-dt RK2 steps, loss =
     L2 misfit of the final density against the run from a seeded perturbed
     initial state. Times (median of `reps`): the plain forward rollout (one
     checkpoint), the forward with ceil(sqrt K) checkpoints, and the whole
     gradient (checkpointed forward + reverse sweep); cost = gradient / plain
     forward. Then checks the gradient against central differences along
     `ndir` seeded smooth directions of the full state at a generic point
-    next to the config-4 state (its rho, u, v, p plus a seeded 1e-3 smooth
-    field each; the config-4 state's exactly uniform u, v, p put every
+    next to the initial state (its rho, u, v, p plus a seeded 1e-3 smooth
+    field each; the initial state's exactly uniform u, v, p put every
     velocity / pressure limiter slope on its kink, where the loss has no
     two-sided derivative)."""
     import numpy as np
@@ -241,17 +241,17 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
             "forward_mcups": round(cells * 2.0 * K / t_fwd / 1e6, 1),
             "tiling": {"forward": {k: v for k, v in fwd_cfg.items() if k in ("1", "2")},
                        "adjoint": {k: v for k, v in adj_cfg.items() if k != "ms_per_step"}},
-            "fd_point": "config-4 state + seeded 1e-3 smooth perturbation of rho, u, v, p",
+            "fd_point": "initial state + seeded 1e-3 smooth perturbation of rho, u, v, p",
             "dt": float(dt).hex(),
             "fd_directions": ndir, "fd_eps": eps, "fd_projected_rel_l2_err": proj_err,
             "fd_eps_scan": scan_err,
             "fd_max_rel_err_per_direction": worst}
 
 
-def config5_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
-    """Config 5: Kelvin-Helmholtz 8192^2 periodic on 64x64 patches of 128^2,
+def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
+    """Kelvin-Helmholtz 8192^2 periodic on 64x64 patches of 128^2,
     `steps` RK2 steps with device dt (graph-replayed); then `nadj`
-    concurrent adjoint gradients of 256^2 instances of the config-4 setup
+    concurrent adjoint gradients of 256^2 instances of the adjoint leg's setup
     (each its own seed) in one batch handle, K = adj_steps, against the same
     gradients one problem at a time."""
     import numpy as np
@@ -346,11 +346,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--per-patch-steps", type=int, default=3)
-    ap.add_argument("--ensemble-steps", type=int, default=100, help="config-3 leg; 0 skips it")
-    ap.add_argument("--adjoint-steps", type=int, default=200, help="config-4 leg; 0 skips it")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="ensemble leg (512 uneven problems); 0 skips it")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="adjoint leg (1024^2, K steps); 0 skips it")
     ap.add_argument("--adjoint-fd-dirs", type=int, default=16,
-                    help="central-difference directions of the config-4 gradient check; 0 skips the check")
-    ap.add_argument("--config5-steps", type=int, default=100, help="config-5 leg (KH 8192^2 + batched adjoints); 0 skips it")
+                    help="central-difference directions of the adjoint gradient check; 0 skips the check")
+    ap.add_argument("--kh-steps", type=int, default=100, help="KH 8192^2 + 64 batched 256^2 adjoints leg; 0 skips it")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling aid: one plain device copy of this many MiB after the legs")
     args = ap.parse_args()
@@ -445,13 +445,13 @@ def main():
 
     adj = None
     if args.adjoint_steps > 0:
-        log("bench: adjoint (config 4)")
+        log("bench: adjoint (1024^2, K steps)")
         adj = adjoint_leg(args.mode, K=args.adjoint_steps, ndir=args.adjoint_fd_dirs)
 
     c5 = None
-    if args.config5_steps > 0:
-        log("bench: config 5 (KH 8192^2, 64 batched 256^2 adjoints)")
-        c5 = config5_leg(args.mode, steps=args.config5_steps)
+    if args.kh_steps > 0:
+        log("bench: KH 8192^2, 64 batched 256^2 adjoints")
+        c5 = kh_leg(args.mode, steps=args.kh_steps)
 
     cpu = None
     if not args.no_cpu:
@@ -505,7 +505,7 @@ def main():
         "per_patch_value": pp,
         "ensemble": ens,
         "adjoint": adj,
-        "config5": c5,
+        "kh_batched_adjoints": c5,
         "roofline": {"bound": "hbm", "kernel": "stage2d_kernel<minmod, 2>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),

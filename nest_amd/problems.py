@@ -118,7 +118,7 @@ def kelvin_helmholtz(n: int, seed=5, gamma=1.4):
 
 
 def ensemble_batch(nprob=512, seed=3, mode="fast", limiter="minmod", Sim2D=None):
-    """Config-3 problem set: `nprob` independent four-quadrant Riemann problems
+    """Ensemble problem set: `nprob` independent four-quadrant Riemann problems
     on s x s cells (s log-uniform in [64, 512]), outflow, all in one batch
     handle. Returns (sim, sizes, U) with U the uploaded halo-filled state."""
     if Sim2D is None:
@@ -135,7 +135,7 @@ def ensemble_batch(nprob=512, seed=3, mode="fast", limiter="minmod", Sim2D=None)
 
 
 def adjoint_setup(n=1024, seed=4, cfl=0.4, perturb=1e-2, gamma=1.4):
-    """Config-4 setup on n x n periodic cells: the smooth initial state
+    """Smooth-state adjoint setup on n x n periodic cells: the initial state
     (smooth_periodic), the initial state of the target run (density times
     1 + perturb*(2 xi - 1), xi ~ U[0,1) per cell from `seed` + 1, energy
     adjusted to keep p = 1), and the fixed dt = cfl*dx / max(|u| + c) of the
@@ -184,8 +184,8 @@ def smooth_direction(n, seed, kmax=4):
 
 
 def adjoint_generic_state(n=1024, seed=4, amp=1e-3, gamma=1.4):
-    """The config-4 state made generic: rho, u, v, p of adjoint_setup's
-    initial state each plus amp times a seeded smooth field. (The config-4
+    """The adjoint setup's state made generic: rho, u, v, p of adjoint_setup's
+    initial state each plus amp times a seeded smooth field. (The setup's
     state itself has exactly uniform u, v, p, so every velocity / pressure
     limiter slope sits on its kink and the loss has no two-sided derivative
     there along directions that move u, v or p.)"""

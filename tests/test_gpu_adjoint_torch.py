@@ -1,22 +1,22 @@
 """GPU: the torch.autograd.Function over the C ABI (nest_amd.adjoint).
 
-Config-4 loss: L2 misfit of the final density after K fixed-dt RK2 steps of
-the smooth periodic state. The adjoint gradient is checked against
+Loss of the smooth-state adjoint workload (1024^2 periodic, K = 200): L2
+misfit of the final density after K fixed-dt RK2 steps of the smooth state. The adjoint gradient is checked against
 * forward-mode dual numbers through the oracle (the same branch decisions as
-  the primal), at the config-4 state itself, along seeded random directions
+  the primal), at the workload's initial state itself, along seeded random directions
   of the full conserved state: relative error <= 1e-10 (exact numerics mode,
   whose primal is bitwise the oracle's);
 * central finite differences along seeded random smooth directions of the
-  full conserved state, at a generic point next to the config-4 state (its
+  full conserved state, at a generic point next to that state (its
   rho, u, v, p plus a seeded 1e-3 smooth field each): the vector of
   directional derivatives (the gradient projected on the directions) within
   1e-6 relative L2.
-  The config-4 state itself is not a point where the loss is differentiable:
+  The initial state itself is not a point where the loss is differentiable:
   its u, v, p are exactly uniform, so every velocity and pressure limiter
   slope has both one-sided differences at rounding level and sits on its
   kink. Central differences there converge to branch averages that no
   gradient reproduces; the agreement measured there (about 1e-4) is printed
-  by the config-4 test, not asserted. Minmod kinks are also dense at a
+  by the full-size test, not asserted. Minmod kinks are also dense at a
   generic point (the central-difference error falls about as sqrt(eps)), so
   the difference step is 1e-9 on directions scaled to max |v| = 1.
 Batched adjoints (many equal-size problems in one handle) must equal the
@@ -101,7 +101,7 @@ def _dual_check(oracle, U0, grad, target, n, npx, npy, K, dt, ndir, seed):
 
 def test_torch_gradient_small(oracle):
     """96^2, 2x3 patches, K = 12, checkpoints every 5 steps: dual numbers at
-    the config-4 state, central differences at the generic point."""
+    the smooth initial state, central differences at the generic point."""
     print_case("torch rollout gradient 96^2 K=12")
     ro, U0, U0d, loss, grad, target, dt = _setup(96, 12, "exact", npx=2, npy=3, interval=5)
     e_dual = _dual_check(oracle, U0, grad, target, 96, 2, 3, 12, dt, 4, 5)
@@ -113,30 +113,30 @@ def test_torch_gradient_small(oracle):
     assert e_fd <= FD_TOL and e_dual <= DUAL_TOL, (e_fd, e_dual)
 
 
-def test_config4_gradient_vs_central_fd():
-    """Config 4 size and rollout: 1024^2 periodic, K = 200 fixed-dt steps,
+def test_adjoint1024_gradient_vs_central_fd():
+    """Full size: 1024^2 periodic, K = 200 fixed-dt steps,
     sqrt(K) = 15-step checkpoint intervals, 16 seeded smooth directions of
     the full state at the generic point (fast numerics). The agreement at
-    the config-4 state itself along density-only directions is printed."""
-    print_case("config 4 gradient vs FD, 1024^2 K=200")
+    the smooth initial state itself along density-only directions is printed."""
+    print_case("adjoint gradient vs FD, 1024^2 K=200")
     ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "fast", npx=4, npy=4,
                                                  U0=P.adjoint_generic_state(1024))
     k, m, _ = ro.sim.rollout_info()
     assert k == 200 and m == 15
     worst = _fd_check(loss, U0d, grad, [P.smooth_direction(1024, 200 + k) for k in range(16)], [FD_EPS, 1e-8])
     ro.close()
-    print("  at the config-4 state itself (density-only directions, not asserted):")
+    print("  at the smooth initial state itself (density-only directions, not asserted):")
     ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "fast", npx=4, npy=4)
     _fd_check(loss, U0d, grad, [P.density_direction(1024, 300 + k) for k in range(2)], [1e-6, 1e-8])
     ro.close()
     assert worst <= FD_TOL, worst
 
 
-def test_config4_gradient_vs_dual_numbers(oracle):
-    """Config 4 at full size, at the config-4 state, against the oracle's
-    dual-number rollout along a random direction of the full conserved
-    state (exact numerics)."""
-    print_case("config 4 gradient vs dual numbers, 1024^2 K=200")
+def test_adjoint1024_gradient_vs_dual_numbers(oracle):
+    """Full size (1024^2, K = 200), at the smooth initial state, against the
+    oracle's dual-number rollout along a random direction of the full
+    conserved state (exact numerics)."""
+    print_case("adjoint gradient vs dual numbers, 1024^2 K=200")
     ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "exact", npx=4, npy=4)
     worst = _dual_check(oracle, U0, grad, target, 1024, 4, 4, 200, dt, 1, 7)
     assert worst <= DUAL_TOL, worst
@@ -144,8 +144,8 @@ def test_config4_gradient_vs_dual_numbers(oracle):
 
 
 def test_batched_adjoints_equal_single():
-    """64 problems of 256^2 in one batch handle (config 5's batched adjoints,
-    config 4's setup per problem with its own seed) vs the same problems one
+    """64 problems of 256^2 in one batch handle (the smooth-state adjoint
+    setup per problem, each with its own seed) vs the same problems one
     at a time: gradients bitwise equal."""
     from nest_amd.adjoint import Rollout, density_misfit
     print_case("batched adjoints 64 x 256^2")

@@ -21,7 +21,7 @@ TOL_FAST = 1e-12
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 @pytest.mark.parametrize("bc", ["outflow", "periodic"])
 def test_1d_sod_lockstep(oracle, mode, bc):
-    """Config 1 (Sod N=1000, CFL 0.4): same U^n and dt fed to both each step."""
+    """1D Sod, N=1000, CFL 0.4: same U^n and dt fed to both each step."""
     n, cfl = 1000, 0.4
     dx = 1.0 / n
     print_case("1d lockstep", mode, bc)
@@ -52,7 +52,7 @@ def test_1d_sod_lockstep(oracle, mode, bc):
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_1d_sod_run_to_t_end(oracle, mode):
-    """Config 1 end to end with device-side dt: step count and final state."""
+    """1D Sod, N=1000, to t = 0.2 with device-side dt: step count and final state."""
     n, cfl, t_end = 1000, 0.4, 0.2
     print_case("1d run", mode)
     U = P.sod1d(n)
@@ -270,11 +270,11 @@ def test_per_patch_path_equals_batched(oracle):
     assert np.array_equal(ga, gb)
 
 
-def test_config2_full_size_one_step(oracle):
-    """Config 2 at full size (4096^2, 16x16 patches of 256^2): one RK2 step
+def test_sod4096_full_size_one_step(oracle):
+    """2D planar Sod at full size (4096^2, 16x16 patches of 256^2): one RK2 step
     bitwise vs the oracle, plus the seeded jitter's mass total."""
     n, npt = 4096, 16
-    print_case("config2 full size")
+    print_case("sod 4096^2 full size")
     sim = Sim2D.mesh(n, n, npt, npt)
     m = MeshSpec(n, n, npt, npt)
     U = sim.scatter(P.sod2d(n, n))
@@ -288,13 +288,13 @@ def test_config2_full_size_one_step(oracle):
     assert np.array_equal(got, U)
 
 
-def test_config5_kh_full_size_one_step(oracle):
-    """Config 5's mesh at full size (Kelvin-Helmholtz 8192^2, periodic, 64x64
+def test_kh8192_full_size_one_step(oracle):
+    """The Kelvin-Helmholtz mesh at full size (8192^2, periodic, 64x64
     patches of 128^2): one RK2 step with the CFL dt, bitwise vs the oracle in
     exact numerics and within 1e-12 in fast numerics; the periodic totals of
     mass and energy are conserved to round-off."""
     n, npt = 8192, 64
-    print_case("config5 KH full size")
+    print_case("KH 8192^2 full size")
     m = MeshSpec(n, n, npt, npt, bcx="periodic", bcy="periodic")
     G = P.kelvin_helmholtz(n)
     U = m.to_patches(G)
@@ -320,12 +320,12 @@ def test_config5_kh_full_size_one_step(oracle):
             assert abs(np.sum(got[k]) - tot0) <= 1e-12 * abs(tot0)
 
 
-def test_config3_ensemble_batched_equals_per_patch_and_oracle(oracle):
-    """Config 3 at full size (512 problems, 64..512 cells a side): 3 batched
+def test_ensemble512_batched_equals_per_patch_and_oracle(oracle):
+    """The 512-problem ensemble at full size (64..512 cells a side): 3 batched
     device-dt steps equal 3 steps of the plain per-patch path bitwise, and a
     sample of problems equals the oracle run alone."""
     from nest_amd import problems as P
-    print_case("config3 ensemble")
+    print_case("ensemble of 512")
     a, sizes, U0 = P.ensemble_batch(512, seed=3, mode="exact")
     b = Sim2D.batch(sizes, mode="exact")
     b.upload(U0)
