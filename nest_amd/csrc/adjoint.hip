@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
 // X kernel
 // ---------------------------------------------------------------------------
 // Two waves per SIMD: the register budget (256 per lane) holds the march's
-// carried state and one face reverse without spilling.
+// carried state and two interleaved face reverses without spilling.
 // OUTX: some problem of the launch has outflow x boundaries (the ghost-row
 // fold is compiled in only then).
 template <int LIM, bool OUTX>
