@@ -25,7 +25,7 @@ import time
 
 import numpy as np
 
-ROWS = (8, 12, 16, 24, 32, 48, 64, 96, 128)
+ROWS = (4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128)
 ADJ_ROWS = (6, 8, 12, 16, 24, 32, 48)
 WAVES = (1, 2, 4)
 

@@ -183,7 +183,16 @@ __device__ __forceinline__ double total_energy(double rho, double u, double p, c
 // velocity (euler2d.hpp:20-33, :115-122). The sound speeds are shared between
 // the two (the reference recomputes the same values, :116-118).
 // F[0..2] = HLLE (mass, normal momentum, energy); F[3] = passive flux.
-template <bool PASSIVE>
+//
+// CLAMP (fast numerics, 2D only): the three branches in one formula, with
+// the speeds clamped as SL- = min(SL, 0), SR+ = max(SR, 0):
+//   F = (SR+ F_L - SL- F_R + SL- SR+ (U_R - U_L)) / (SR+ - SL-)
+// is F_L when SL >= 0 (SL- = 0, the quotient SR F_L / SR), F_R when SR <= 0,
+// and the HLLE average otherwise; likewise the interface velocity. It
+// replaces the branch selects by two min/max and differs from the selected
+// value by the rounding of SR * (1/SR) (a few ulp). The denominator is
+// positive: SR - SL >= c_L + c_R > 0, as the 2D floors keep rho, p > 0.
+template <bool PASSIVE, bool CLAMP = false>
 __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const eos_c& e, double F[4]) {
     double cL = PASSIVE ? sound_speed(L.r, L.p, e) : sound_speed_1d(L.r, L.p, e);
     double EL = total_energy(L.r, L.u, L.p, e);
@@ -194,6 +203,22 @@ __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const 
     double FL0 = L.r * L.u, FL1 = L.r * L.u * L.u + L.p, FL2 = (EL + L.p) * L.u;
     double FR0 = R.r * R.u, FR1 = R.r * R.u * R.u + R.p, FR2 = (ER + R.p) * R.u;
     double UL1 = L.r * L.u, UR1 = R.r * R.u;
+#ifndef NEST_HLL_CLAMP
+#define NEST_HLL_CLAMP 1
+#endif
+#if NEST_FAST && NEST_HLL_CLAMP
+    if constexpr (CLAMP && PASSIVE) {
+        const double SLm = dmin(SL, 0.0), SRp = dmax(SR, 0.0);
+        const double inv = rcp(SRp - SLm);
+        const double s2 = SLm * SRp;
+        F[0] = (SRp * FL0 - SLm * FR0 + s2 * (R.r - L.r)) * inv;
+        F[1] = (SRp * FL1 - SLm * FR1 + s2 * (R.r * R.u - L.r * L.u)) * inv;
+        F[2] = (SRp * FL2 - SLm * FR2 + s2 * (ER - EL)) * inv;
+        const double ui = (SRp * L.u - SLm * R.u) * inv;
+        F[3] = ui * ((ui >= 0.0) ? L.q : R.q);
+        return;
+    }
+#endif
     double sLR = SL * SR;
 #if NEST_FAST
     double inv = rcp(SR - SL);
@@ -257,6 +282,19 @@ __device__ __forceinline__ double lane_down(double x) {
     int lo = __double2loint(x), hi = __double2hiint(x);
     lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
     hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// The same shifts with lane 0 (lane_up_z) / lane 63 (lane_down_z) receiving
+// 0 instead of keeping its own value: no old operand, so no register copy
+// before the DPP move. For stencils whose edge-lane results are not used.
+__device__ __forceinline__ double lane_up_z(double x) {
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_down_z(double x) {
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x130, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 #endif

@@ -4,10 +4,10 @@
 // (-ffp-contract=off, IEEE division and sqrt: bitwise the CPU numerics) and
 // NEST_FAST=1 / namespace nest_fast (FMA contraction, reciprocal division).
 //
-// 2D stage kernel ("strip march"): one wave owns a strip of 64 consecutive
+// 2D stage kernels ("strip march"): one wave owns a strip of 64 consecutive
 // extended columns (dim 1, contiguous in memory) of one patch and marches
 // down interior rows [r0, r1) of dim 0. Per row it loads one new row of the
-// conserved state (four coalesced 512 B loads), and:
+// conserved state (4 variables x 512 B), and:
 //   * X-sweep (along the march): primitives, differences, slopes and the
 //     face flux are carried in registers as a sliding window, so every
 //     cell's primitives and every x-face flux are computed exactly once;
@@ -17,9 +17,14 @@
 //     stencil halo (the X-sweep of the halo columns is exactly what the
 //     split scheme's halo exchange would have delivered);
 //   * epilogue: stage 1 stores U1; stage 2 stores 0.5*(U^n + S(U1)) in place
-//     of U^n and folds |u|+c, |v|+c of the new state into the problem's
-//     wave-speed maxima (u64 bit-pattern atomicMax: exact, order-free).
-// No LDS is used and nothing but the state itself touches HBM.
+//     of U^n and folds |u|+c, |v|+c of the new state into one of the
+//     problem's wave-speed slots (u64 bit-pattern atomicMax: exact,
+//     order-free); the dt finalize folds the slots.
+// Two forms: stage2d_dma_kernel (rows fetched straight into a per-wave LDS
+// ring, used whenever a state buffer is below 4 GiB) and stage2d_kernel
+// (rows loaded into registers, with an LDS ring for the centre rows; 64-bit
+// addressing, used above that size). Nothing but the state itself touches
+// HBM.
 #include "device_numerics.hpp"
 #include "kernels.hpp"
 
@@ -54,6 +59,13 @@ __device__ __forceinline__ prim4 shfl_up4(const prim4& a) {
 }
 __device__ __forceinline__ prim4 shfl_down4(const prim4& a) {
     return {lane_down(a.r), lane_down(a.u), lane_down(a.p), lane_down(a.q)};
+}
+
+__device__ __forceinline__ prim4 shfl_up4_z(const prim4& a) {
+    return {lane_up_z(a.r), lane_up_z(a.u), lane_up_z(a.p), lane_up_z(a.q)};
+}
+__device__ __forceinline__ prim4 shfl_down4_z(const prim4& a) {
+    return {lane_down_z(a.r), lane_down_z(a.u), lane_down_z(a.p), lane_down_z(a.q)};
 }
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
@@ -336,8 +348,309 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             smx = wave_max_u64(smx);
             smy = wave_max_u64(smy);
             if (lane == 0) {
-                atomicMax(&ps->smax[0], smx);
-                atomicMax(&ps->smax[1], smy);
+                unsigned long long* sl =
+                    a.slots + ((size_t)pd.prob * NEST_SPEED_SLOTS + (it & (NEST_SPEED_SLOTS - 1))) * NEST_SLOT_STRIDE;
+                atomicMax(sl, smx);
+                atomicMax(sl + 1, smy);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel with the row loads going straight into LDS.
+//
+// Same strip march and arithmetic as stage2d_kernel, but every row of the
+// input state (and, in stage 2, every row of U^n at the output cells) is
+// fetched with global_load_lds_dword into a per-wave LDS ring instead of
+// into registers. Nothing loaded is carried in VGPRs across the march, so no
+// register copy at the loop back-edge waits on a load in flight, and each
+// row is requested two march steps before it is used (stage 2's U^n row one
+// step before), with explicit vmcnt waits that count only the memory
+// operations known to be issued after the awaited one.
+//
+// Dword view of a row: the 64 doubles of one variable are 512 contiguous
+// bytes of LDS, filled by two instructions of 64 lanes x 4 B. Lane l of the
+// first fetches half (l & 1) of the strip's double (l >> 1), of the second
+// half (l & 1) of double 32 + (l >> 1). The source addresses are those the
+// register path would use for those doubles (shuffled from the owning lane),
+// so every access stays inside the state buffers.
+//
+// Per wave, LDS: 4 row slots x 4 variables x 64 doubles (8 KB); stage 2 adds
+// one slot for the U^n row (2 KB). Dynamic shared memory: waves per block x
+// that (4 waves of stage 2 per SIMD: 16 x 10 KB = the CU's 160 KB).
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// s_waitcnt (gfx9 encoding: vmcnt[3:0] in bits 3:0 and vmcnt[5:4] in 15:14,
+// expcnt in 6:4, lgkmcnt in 11:8). The signal fences keep the compiler from
+// moving memory operations across the wait.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void wait_lgkmcnt0() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void compiler_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+__device__ __forceinline__ const char* shfl_cptr(const void* p, int src) {
+    return reinterpret_cast<const char*>(__shfl(reinterpret_cast<long long>(p), src));
+}
+
+constexpr int kRowSlot = 4 * 64;  // doubles per ring slot (4 variables x 64 columns)
+constexpr int kWaveLds1 = 4 * kRowSlot;                 // stage 1: 4 row slots
+constexpr int kWaveLds2 = 4 * kRowSlot + kRowSlot;  // stage 2: + 1 U^n slot
+
+// 4 waves per SIMD: both stages fit 128 VGPRs, and stage 2's LDS (10 KB per
+// wave) fills the CU's 160 KB exactly
+template <int LIM, int STAGE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage2d_dma_kernel(stage_args a) {
+    extern __shared__ double lds_dyn[];
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    // the wave's index within the block, as a wave-uniform value: the work
+    // item, its patch and rows, and the LDS slot addresses then live in
+    // scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr int kWave = STAGE == 1 ? kWaveLds1 : kWaveLds2;
+    double* const ring = lds_dyn + wave * kWave;  // row slot of extended row e: (e & 3)
+    double* const u0slot = ring + 4 * kRowSlot;   // stage 2: U^n row of the output row being marched
+    const int hl = lane >> 1;
+    const int part = (lane & 1) * 4;
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
+        const nest_item w = a.items[it];
+        const nest_patch_desc pd = a.patches[w.patch];
+        nest_prob_state* ps = a.probs + pd.prob;
+        if (!(a.host_dt > 0.0) && ps->done) continue;
+        const double dt = a.host_dt > 0.0 ? a.host_dt : ps->dt;
+        const double dtdx = dt / ps->dx;
+        const double dtdy = dt / ps->dy;
+
+        const double* Ubuf = STAGE == 1 ? a.U : a.U1;
+        double* Obuf = STAGE == 1 ? a.U1 : (a.Uo ? a.Uo : a.U);
+        const char* const Ub8 = reinterpret_cast<const char*>(Ubuf);
+        const char* const U08 = reinterpret_cast<const char*>(a.U);  // stage 2: U^n at the output cells
+        char* const Ob8 = reinterpret_cast<char*>(Obuf);
+        // Per lane: byte offset (in every state buffer alike: they share the
+        // patch layout and are below 4 GiB) of this lane's column at extended
+        // row 0, its row length and variable stride in bytes.
+        unsigned coff;
+        int snye, svs;
+        bool store;
+        int g = 0;
+        nest_prob_geom G{};
+        if (a.gather) {
+            G = a.geoms[pd.prob];
+            g = pd.y0 + w.c0 + lane - 2;
+            int q, le, lc;
+            nest_gather_source(pd, G, a.itab, 2, g, q, le, lc);
+            const nest_patch_desc qd = a.patches[q];
+            snye = (qd.ny + 4) * 8;
+            svs = (qd.nx + 4) * (qd.ny + 4) * 8;
+            coff = (unsigned)((qd.off + lc) * 8);
+            store = lane >= 2 && lane < 62 && g < G.NY;
+        } else {
+            const int nye = pd.ny + 4;
+            const int ec = w.c0 + lane;
+            const int ecl = ec < nye ? ec : nye - 1;
+            snye = nye * 8;
+            svs = (pd.nx + 4) * nye * 8;
+            coff = (unsigned)((pd.off + ecl) * 8);
+            store = lane >= 2 && lane < 62 && ec >= 2 && ec < pd.ny + 2;
+        }
+        // the dword view's source columns: strip doubles hl and 32 + hl
+        const unsigned bA = (unsigned)__shfl((int)coff, hl) + part;
+        const unsigned bB = (unsigned)__shfl((int)coff, 32 + hl) + part;
+        const int rsA = __shfl(snye, hl), rsB = __shfl(snye, 32 + hl);
+        const int vsA = __shfl(svs, hl), vsB = __shfl(svs, 32 + hl);
+
+        // row min(e, last extended row) of the input state into ring slot
+        // (e & 3): 8 memory operations
+        const int elast = pd.nx + 3;
+        auto issue_row = [&](int e_slot) {
+            const int e = min(e_slot, elast);
+            unsigned oA, oB;
+            int sA, sB;
+            if (a.gather && (e < 2 || e >= pd.nx + 2)) {  // ghost rows: wave-uniform slow path
+                int q, le, lc;
+                nest_gather_source(pd, G, a.itab, e, g, q, le, lc);
+                const nest_patch_desc qd = a.patches[q];
+                const int qnye = qd.ny + 4;
+                const unsigned o = (unsigned)((qd.off + (long long)le * qnye + lc) * 8);
+                const int s8 = (qd.nx + 4) * qnye * 8;
+                oA = (unsigned)__shfl((int)o, hl) + part;
+                oB = (unsigned)__shfl((int)o, 32 + hl) + part;
+                sA = __shfl(s8, hl);
+                sB = __shfl(s8, 32 + hl);
+            } else {
+                oA = bA + (unsigned)e * (unsigned)rsA;
+                oB = bB + (unsigned)e * (unsigned)rsB;
+                sA = vsA;
+                sB = vsB;
+            }
+            double* slot = ring + (e_slot & 3) * kRowSlot;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                __builtin_amdgcn_global_load_lds(Ub8 + (oA + (unsigned)k * (unsigned)sA), (lds_void_ptr)(slot + k * 64), 4, 0,
+                                                 0);
+                __builtin_amdgcn_global_load_lds(Ub8 + (oB + (unsigned)k * (unsigned)sB), (lds_void_ptr)(slot + k * 64 + 32),
+                                                 4, 0, 0);
+            }
+        };
+        // stage 2: U^n of output row min(i, r1 - 1) (extended row + 2) into
+        // the U^n slot: 8 memory operations
+        auto issue_u0 = [&](int i_want) {
+            const int i = min(i_want, w.r1 - 1);
+            const unsigned oA = bA + (unsigned)(i + 2) * (unsigned)rsA;
+            const unsigned oB = bB + (unsigned)(i + 2) * (unsigned)rsB;
+            double* slot = u0slot;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                __builtin_amdgcn_global_load_lds(U08 + (oA + (unsigned)k * (unsigned)vsA), (lds_void_ptr)(slot + k * 64), 4, 0,
+                                                 0);
+                __builtin_amdgcn_global_load_lds(U08 + (oB + (unsigned)k * (unsigned)vsB), (lds_void_ptr)(slot + k * 64 + 32),
+                                                 4, 0, 0);
+            }
+        };
+        auto read_row = [&](int e, double* u) {
+            const double* q = ring + (e & 3) * kRowSlot + lane;
+            u[0] = q[0];
+            u[1] = q[64];
+            u[2] = q[128];
+            u[3] = q[192];
+        };
+
+        // ---- prologue: rows r0 .. r0+3 (X window warm-up), then the first
+        // two prefetches (rows r0+4, r0+5) and stage 2's first U^n row
+        issue_row(w.r0);
+        issue_row(w.r0 + 1);
+        issue_row(w.r0 + 2);
+        issue_row(w.r0 + 3);
+        wait_vmcnt<0>();
+        prim4 W1, dW1, FL;
+        double Fm[4];
+        {
+            double A[4];
+            read_row(w.r0, A);
+            prim4 Wm2 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            read_row(w.r0 + 1, A);
+            prim4 Wm1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            read_row(w.r0 + 2, A);
+            prim4 W0 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            read_row(w.r0 + 3, A);
+            W1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            wait_lgkmcnt0();  // slots of rows r0, r0+1 are refilled next
+            issue_row(w.r0 + 4);
+            compiler_fence();
+            issue_row(w.r0 + 5);
+            if constexpr (STAGE == 2) {
+                compiler_fence();
+                issue_u0(w.r0);
+            }
+            prim4 d0 = sub4(W0, Wm1);
+            dW1 = sub4(W1, W0);
+            prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
+            prim4 s0 = slope4<LIM>(d0, dW1);
+            face_flux<true, true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+            FL = face_plus(W0, s0);
+        }
+
+        unsigned long long smx = 0, smy = 0;
+        for (int i = w.r0; i < w.r1; ++i) {
+            // Row i+4 was issued two steps ago (prologue for i < r0+2); issued
+            // after it for sure: the whole previous step's fetches (row i+5,
+            // and in stage 2 the U^n rows issued after it), or the
+            // prologue's later groups.
+            if constexpr (STAGE == 1) wait_vmcnt<8>();
+            else wait_vmcnt<16>();
+            double Uc[4], Ub[4];
+            read_row(i + 2, Uc);
+            read_row(i + 4, Ub);
+            wait_lgkmcnt0();  // the centre row's slot is refilled next
+            issue_row(i + 6);
+
+            // X step of row i (x_sweep_t update, euler2d.hpp:144-148)
+            double ux[4];
+            {
+                prim4 W2 = prim2d(Ub[0], Ub[1], Ub[2], Ub[3], true, a.eos);
+                prim4 dW2 = sub4(W2, W1);
+                prim4 s1 = slope4<LIM>(dW1, dW2);
+                double Fp[4];
+                face_flux<true, true>(FL, face_minus(W1, s1), a.eos, Fp);
+                FL = face_plus(W1, s1);
+                ux[0] = Uc[0] - dtdx * (Fp[0] - Fm[0]);
+                ux[1] = Uc[1] - dtdx * (Fp[1] - Fm[1]);
+                ux[2] = Uc[2] - dtdx * (Fp[3] - Fm[3]);
+                ux[3] = Uc[3] - dtdx * (Fp[2] - Fm[2]);
+                W1 = W2;
+                dW1 = dW2;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Fm[k] = Fp[k];
+            }
+            // Y step of row i (euler2d.hpp:176-236). The values lanes 0 and
+            // 63 receive from outside the wave (zero) reach only lanes 0, 1,
+            // 62 and 63, which store nothing.
+            prim4 Y = prim2d(ux[0], ux[1], ux[2], ux[3], false, a.eos);
+            prim4 dl = sub4(Y, shfl_up4_z(Y));
+            prim4 dr = shfl_down4_z(dl);
+            prim4 sy = slope4<LIM>(dl, dr);
+            prim4 Rf = shfl_down4_z(face_minus(Y, sy));
+            double Gf[4];
+            face_flux<true, true>(face_plus(Y, sy), Rf, a.eos, Gf);
+            double Gm[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Gm[k] = lane_up_z(Gf[k]);
+            double o0 = ux[0] - dtdy * (Gf[0] - Gm[0]);
+            double o1 = ux[1] - dtdy * (Gf[3] - Gm[3]);
+            double o2 = ux[2] - dtdy * (Gf[1] - Gm[1]);
+            double o3 = ux[3] - dtdy * (Gf[2] - Gm[2]);
+            if constexpr (STAGE == 2) {
+                // U^n row i was issued at the end of the previous step
+                // (prologue for i = r0); issued after it for sure: this
+                // step's row fetch
+                wait_vmcnt<8>();
+                const double* q = u0slot + lane;
+                const double z0 = q[0], z1 = q[64], z2 = q[128], z3 = q[192];
+                wait_lgkmcnt0();  // the slot is refilled next
+                issue_u0(i + 1);
+                // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2)
+                o0 = 0.5 * (z0 + o0);
+                o1 = 0.5 * (z1 + o1);
+                o2 = 0.5 * (z2 + o2);
+                o3 = 0.5 * (z3 + o3);
+                double sx, sy2;
+                cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
+                unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+                unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
+                smx = (store && bx > smx) ? bx : smx;
+                smy = (store && by > smy) ? by : smy;
+            }
+            if (store) {
+                const unsigned o = coff + (unsigned)(i + 2) * (unsigned)snye;
+                *reinterpret_cast<double*>(Ob8 + o) = o0;
+                *reinterpret_cast<double*>(Ob8 + (o + (unsigned)svs)) = o1;
+                *reinterpret_cast<double*>(Ob8 + (o + 2u * (unsigned)svs)) = o2;
+                *reinterpret_cast<double*>(Ob8 + (o + 3u * (unsigned)svs)) = o3;
+            }
+        }
+        // the last steps' prefetches must land before the ring is reused by
+        // the next work item (or the wave ends)
+        wait_vmcnt<0>();
+        if constexpr (STAGE == 2) {
+            smx = wave_max_u64(smx);
+            smy = wave_max_u64(smy);
+            if (lane == 0) {
+                unsigned long long* sl =
+                    a.slots + ((size_t)pd.prob * NEST_SPEED_SLOTS + (it & (NEST_SPEED_SLOTS - 1))) * NEST_SLOT_STRIDE;
+                atomicMax(sl, smx);
+                atomicMax(sl + 1, smy);
             }
         }
     }
@@ -403,17 +716,34 @@ __global__ __launch_bounds__(256) void fill_regions_kernel(double* base, const n
 //    is monotone), init 1e10, cells with s == 0 skipped;
 // 1D compute_dt (sod1d.cpp:127): (cfl*dx)/max(|u|+c);
 // then dt = min(dt, t_end - t) (sod1d.cpp:222, sod2d.cpp:320).
-__global__ void finalize_kernel(nest_prob_state* P, int nprob) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
+    const int i = blockIdx.x;  // one wave per problem
+    const int lane = threadIdx.x;
     if (i >= nprob) return;
+    unsigned long long vx = 0, vy = 0;
+    if (slots) {  // fold and clear the stage-2 slots (lane k: slot k)
+        unsigned long long* sl = slots + ((size_t)i * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
+        vx = sl[0];
+        vy = sl[1];
+        sl[0] = 0ull;
+        sl[1] = 0ull;
+    }
+    vx = wave_max_u64(vx);
+    vy = wave_max_u64(vy);
+    if (lane != 0) return;
     nest_prob_state& s = P[i];
+    // speeds2d / the 1D kernels reduce straight into smax
+    const unsigned long long bx = s.smax[0] > vx ? s.smax[0] : vx;
+    const unsigned long long by = s.smax[1] > vy ? s.smax[1] : vy;
+    s.smax[0] = 0ull;
+    s.smax[1] = 0ull;
     if (s.done) return;
     if (!(s.t < s.t_end)) {
         s.done = 1;
         return;
     }
-    const double sx = __longlong_as_double((long long)s.smax[0]);
-    const double sy = __longlong_as_double((long long)s.smax[1]);
+    const double sx = __longlong_as_double((long long)bx);
+    const double sy = __longlong_as_double((long long)by);
     double dt;
     if (s.fixed_dt > 0.0) {
         dt = s.fixed_dt;
@@ -434,13 +764,18 @@ __global__ void finalize_kernel(nest_prob_state* P, int nprob) {
     s.dt = dt;
     s.t = s.t + dt;
     s.steps += 1;
-    s.smax[0] = 0ull;
-    s.smax[1] = 0ull;
 }
 
-__global__ void reset_speeds_kernel(nest_prob_state* P, int nprob) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nprob) {
+__global__ __launch_bounds__(64) void reset_speeds_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= nprob) return;
+    if (slots) {
+        unsigned long long* sl = slots + ((size_t)i * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
+        sl[0] = 0ull;
+        sl[1] = 0ull;
+    }
+    if (lane == 0) {
         P[i].smax[0] = 0ull;
         P[i].smax[1] = 0ull;
     }
@@ -538,8 +873,23 @@ __global__ void fill1d_kernel(double* U, int n, int bc, const nest_prob_state* p
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+#ifndef NEST_STAGE_DMA
+#define NEST_STAGE_DMA 1
+#endif
 template <int STAGE>
 static void stage2d_dispatch(int lim, const stage_args& a, int grid, int block, hipStream_t s) {
+#if NEST_STAGE_DMA
+    // the LDS-ring kernel addresses the state buffers with 32-bit byte offsets
+    if (a.total > 0 && a.total * (long long)sizeof(double) < (1LL << 32)) {
+    const size_t lds = (size_t)(block / 64) * (STAGE == 1 ? kWaveLds1 : kWaveLds2) * sizeof(double);
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((stage2d_dma_kernel<LIM_MC, STAGE>), grid, block, lds, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((stage2d_dma_kernel<LIM_NONE, STAGE>), grid, block, lds, s, a); break;
+        default: hipLaunchKernelGGL((stage2d_dma_kernel<LIM_MINMOD, STAGE>), grid, block, lds, s, a); break;
+    }
+    return;
+    }
+#endif
     switch (lim) {
         case LIM_MC: hipLaunchKernelGGL((stage2d_kernel<LIM_MC, STAGE>), grid, block, 0, s, a); break;
         case LIM_NONE: hipLaunchKernelGGL((stage2d_kernel<LIM_NONE, STAGE>), grid, block, 0, s, a); break;
@@ -561,12 +911,14 @@ void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars
     hipLaunchKernelGGL(fill_regions_kernel, nreg, 256, 0, s, base, R, nreg, nvars);
 }
 
-void launch_finalize(nest_prob_state* P, int nprob, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_kernel, (nprob + 255) / 256, 256, 0, s, P, nprob);
+void launch_finalize(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s) {
+    if (nprob <= 0) return;
+    hipLaunchKernelGGL(finalize_kernel, nprob, NEST_SPEED_SLOTS, 0, s, P, slots, nprob);
 }
 
-void launch_reset_speeds(nest_prob_state* P, int nprob, hipStream_t s) {
-    hipLaunchKernelGGL(reset_speeds_kernel, (nprob + 255) / 256, 256, 0, s, P, nprob);
+void launch_reset_speeds(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s) {
+    if (nprob <= 0) return;
+    hipLaunchKernelGGL(reset_speeds_kernel, nprob, NEST_SPEED_SLOTS, 0, s, P, slots, nprob);
 }
 
 template <int STAGE>

@@ -12,8 +12,8 @@
     void launch_stage2d(int stage, int lim, const stage_args& a, int grid, int block, hipStream_t s); \
     void launch_speeds2d(const stage_args& a, int grid, int block, hipStream_t s);                  \
     void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars, hipStream_t s); \
-    void launch_finalize(nest_prob_state* P, int nprob, hipStream_t s);                             \
-    void launch_reset_speeds(nest_prob_state* P, int nprob, hipStream_t s);                         \
+    void launch_finalize(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s);  \
+    void launch_reset_speeds(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s); \
     void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s);                  \
     void launch_speeds1d(const stage1d_args& a, hipStream_t s);                                     \
     void launch_fill1d(double* U, int n, int bc, const nest_prob_state* ps, hipStream_t s);         \
@@ -41,6 +41,8 @@ struct stage_args {
     const nest_prob_geom* geoms;
     const int* itab;
     double* Uo;  // stage 2 output (null: in place of U)
+    long long total;  // doubles in each state buffer (all share the patch layout)
+    unsigned long long* slots;  // stage 2: per-problem wave-speed slots (NEST_SPEED_SLOTS each)
 };
 
 // arguments of the two adjoint kernels of one split stage S = Y o X (the

@@ -45,8 +45,8 @@ struct launch_table {
     void (*stage2d)(int, int, const stage_args&, int, int, hipStream_t);
     void (*speeds2d)(const stage_args&, int, int, hipStream_t);
     void (*fill_regions)(double*, const nest_region*, int, int, hipStream_t);
-    void (*finalize)(nest_prob_state*, int, hipStream_t);
-    void (*reset_speeds)(nest_prob_state*, int, hipStream_t);
+    void (*finalize)(nest_prob_state*, unsigned long long*, int, hipStream_t);
+    void (*reset_speeds)(nest_prob_state*, unsigned long long*, int, hipStream_t);
     void (*stage1d)(int, int, const stage1d_args&, hipStream_t);
     void (*speeds1d)(const stage1d_args&, hipStream_t);
     void (*fill1d)(double*, int, int, const nest_prob_state*, hipStream_t);
@@ -137,6 +137,7 @@ struct nest_sim2d {
     nest_region* regions_d = nullptr;
     nest_item* items_d = nullptr;
     nest_prob_state* probs_d = nullptr;
+    unsigned long long* slots_d = nullptr;  // stage-2 wave-speed slots, NEST_SPEED_SLOTS per problem
     nest_item* gitems_d[2] = {nullptr, nullptr};
     nest_prob_geom* geoms_d = nullptr;
     int* itab_d = nullptr;
@@ -187,7 +188,7 @@ struct nest_sim2d {
     void release() {
         drop_graph();
         for (void* p : {(void*)U_own, (void*)U1_own, (void*)patches_d, (void*)regions_d, (void*)items_d,
-                        (void*)probs_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)geoms_d, (void*)itab_d,
+                        (void*)probs_d, (void*)slots_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)geoms_d, (void*)itab_d,
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
                         (void*)goff_d})
             if (p) (void)hipFree(p);
@@ -218,6 +219,7 @@ struct nest_sim2d {
         regions_d = nullptr;
         items_d = nullptr;
         probs_d = nullptr;
+        slots_d = nullptr;
         if (own_stream) (void)hipStreamDestroy(own_stream);
         own_stream = nullptr;
     }
@@ -235,11 +237,13 @@ struct nest_sim2d {
         a.items = (gather ? gitems_d[gather - 1] : items_d) + item0;
         a.nitems = nitems;
         a.probs = probs_d;
+        a.slots = slots_d;
         a.eos = eos;
         a.host_dt = host_dt;
         a.gather = gather;
         a.geoms = geoms_d;
         a.itab = itab_d;
+        a.total = (long long)total;
         return a;
     }
     int grid_for(int nitems, int waves) const {
@@ -312,6 +316,9 @@ struct nest_sim2d {
         if (int rc = dev_alloc(&patches_d, patches_h.size())) return rc;
         if (int rc = dev_alloc(&regions_d, regions_h.size())) return rc;
         if (int rc = dev_alloc(&probs_d, probs_h.size())) return rc;
+        const size_t nslot = probs_h.size() * (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE;
+        if (int rc = dev_alloc(&slots_d, nslot)) return rc;
+        NEST_HIP_TRY(hipMemset(slots_d, 0, nslot * sizeof(unsigned long long)));
         if (int rc = dev_alloc(&geoms_d, tb.geoms.size())) return rc;
         if (int rc = dev_alloc(&itab_d, tb.itab.size())) return rc;
         NEST_HIP_TRY(hipMemcpy(geoms_d, tb.geoms.data(), tb.geoms.size() * sizeof(nest_prob_geom),
@@ -337,7 +344,7 @@ struct nest_sim2d {
         int dev = 0, cus = 0;
         NEST_HIP_TRY(hipGetDevice(&dev));
         NEST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        resident_waves = cus * 4 * 3;  // 4 SIMDs per CU, ~3 waves/SIMD at the stage kernel's VGPR use
+        resident_waves = cus * 4 * 4;  // 4 SIMDs per CU, 4 waves/SIMD (the LDS-ring stage kernels)
         return build_items();
     }
 
@@ -362,7 +369,7 @@ struct nest_sim2d {
     // from the neighbours (no exchange launches). Ghost cells of U are left
     // stale; callers refresh them with fill(U) when they hand U out.
     void device_step() {
-        L().finalize(probs_d, nprob, stream);
+        L().finalize(probs_d, slots_d, nprob, stream);
         gstage(1, 0.0);
         gstage(2, 0.0);
     }
@@ -702,7 +709,7 @@ int nest_hip_sim2d_exchange(nest_sim2d* s, int buffer) {
 int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host) {
     if (int rc = valid_sim(s)) return rc;
     if (!dt_host) return fail(NEST_ERR_INVALID, "dt_host is null");
-    s->L().reset_speeds(s->probs_d, s->nprob, s->stream);
+    s->L().reset_speeds(s->probs_d, s->slots_d, s->nprob, s->stream);
     s->speeds(0, (int)s->items_h.size());
     NEST_HIP_TRY(hipGetLastError());
     NEST_HIP_TRY(hipMemcpyAsync(s->probs_h.data(), s->probs_d, s->probs_h.size() * sizeof(nest_prob_state),
@@ -747,6 +754,8 @@ int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_d
     }
     NEST_HIP_TRY(hipMemcpyAsync(s->probs_d, s->probs_h.data(), s->probs_h.size() * sizeof(nest_prob_state),
                                 hipMemcpyHostToDevice, s->stream));
+    // stage-2 slots may hold maxima of earlier host-dt stages
+    s->L().reset_speeds(s->probs_d, s->slots_d, s->nprob, s->stream);
     s->speeds(0, (int)s->items_h.size());
     NEST_HIP_TRY(hipGetLastError());
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
@@ -796,7 +805,7 @@ int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl) {
     std::vector<double> dts(s->nprob);
     for (int k = 0; k < nsteps; ++k) {
         // Reduce: wave speeds patch by patch, blocking read-back, host dt
-        s->L().reset_speeds(s->probs_d, s->nprob, s->stream);
+        s->L().reset_speeds(s->probs_d, s->slots_d, s->nprob, s->stream);
         for (int p = 0; p < np; ++p) s->speeds(s->item_begin[p], s->item_end[p] - s->item_begin[p]);
         NEST_HIP_TRY(hipMemcpyAsync(s->probs_h.data(), s->probs_d, s->probs_h.size() * sizeof(nest_prob_state),
                                     hipMemcpyDeviceToHost, s->stream));
@@ -828,7 +837,7 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
         for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(), with events between
             hipEvent_t* e = &ev[4 * (size_t)k];
             (void)hipEventRecord(e[0], s->stream);
-            s->L().finalize(s->probs_d, s->nprob, s->stream);
+            s->L().finalize(s->probs_d, s->slots_d, s->nprob, s->stream);
             (void)hipEventRecord(e[1], s->stream);
             s->gstage(1, 0.0);
             (void)hipEventRecord(e[2], s->stream);
@@ -1100,7 +1109,7 @@ int nest_hip_solver1d_download(nest_solver1d* s, double* host) {
 
 int nest_hip_solver1d_compute_dt(nest_solver1d* s, double cfl, double* dt) {
     if (!s || !dt) return fail(NEST_ERR_INVALID, "null handle or dt");
-    s->L().reset_speeds(s->ps_d, 1, s->stream);
+    s->L().reset_speeds(s->ps_d, nullptr, 1, s->stream);
     s->L().speeds1d(s->args(0.0), s->stream);
     NEST_HIP_TRY(hipGetLastError());
     NEST_HIP_TRY(hipMemcpyAsync(&s->ps_h, s->ps_d, sizeof(nest_prob_state), hipMemcpyDeviceToHost, s->stream));
@@ -1131,7 +1140,7 @@ int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* ste
     const int batch = 64;
     for (;;) {
         for (int k = 0; k < batch; ++k) {
-            s->L().finalize(s->ps_d, 1, s->stream);
+            s->L().finalize(s->ps_d, nullptr, 1, s->stream);
             s->step(0.0);
         }
         NEST_HIP_TRY(hipGetLastError());

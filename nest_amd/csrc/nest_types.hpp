@@ -33,6 +33,15 @@ struct nest_prob_geom {
     int tab;
 };
 
+// Wave-speed maxima of the 2D stage-2 kernels: every work item folds its
+// maxima into one of NEST_SPEED_SLOTS per-problem slots (item index modulo
+// the slot count), each slot on its own 128-byte line, so concurrent items
+// do not queue on one address; the dt finalize folds the slots (one lane
+// each) and clears them. Layout: slots[(prob * NEST_SPEED_SLOTS + k) *
+// NEST_SLOT_STRIDE + {0: |u|+c, 1: |v|+c}], u64 bit patterns.
+constexpr int NEST_SPEED_SLOTS = 64;
+constexpr int NEST_SLOT_STRIDE = 16;
+
 // Per-problem time-stepping state, resident on the device so a whole step
 // (dt reduction included) runs without a host round trip.
 struct nest_prob_state {
