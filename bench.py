@@ -506,7 +506,7 @@ def main():
         "ensemble": ens,
         "adjoint": adj,
         "kh_batched_adjoints": c5,
-        "roofline": {"bound": "hbm", "kernel": "stage2d_kernel<minmod, 2>", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "stage2d_dma_kernel<minmod, 2>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),
                      "measured_copy_peak": round(copy_gbs, 1), "frac_vs_copy_peak": round(achieved / copy_gbs, 4),

@@ -130,6 +130,13 @@ int nest_hip_sim2d_device_ptrs(nest_sim2d* s, double** dev_U, double** dev_U1);
    0 keeps the current value. persistent != 0 caps the grid at the number of
    resident waves and loops over the work list. */
 int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int persistent);
+/* Which 2D stage kernel runs: 0 automatic (the LDS-ring kernel, which
+   fetches rows straight into LDS and addresses the state buffers with 32-bit
+   offsets, whenever a state buffer is below 4 GiB; else the register-path
+   kernel), 1 register path, 2 LDS-ring path (NEST_ERR_UNSUPPORTED at 4 GiB
+   and above). Both compute the same arithmetic. No counterpart in the
+   reference: a backend choice, kept selectable so both paths stay tested. */
+int nest_hip_sim2d_set_stage_kernel(nest_sim2d* s, int kind);
 /* Tiling of one fused stage (1 or 2) of the device-dt / rk2_step path only;
    0 keeps the current value. get_tiling(): rows[2], waves_per_block[2] of
    the two fused stages (either pointer may be NULL). */
@@ -207,7 +214,9 @@ int nest_hip_sim2d_adjoint_stage(nest_sim2d* s, const double* U, const double* G
 int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double dt);
 /* nsteps fixed-dt RK2 steps of the handle's state, storing a checkpoint every
    `interval` steps (<= 0: ceil(sqrt(nsteps))) in device buffers owned by the
-   handle; the final state's halos are refilled. */
+   handle; the final state's halos are refilled. With two or more intervals,
+   the states of the last interval and their stage-1 states are kept too (in
+   the buffers the reverse sweep reads), so its reverse needs no recompute. */
 int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval);
 /* Reverse of the last rollout: Ubar (in: cotangent of the final state, out:
    cotangent of the initial state), recomputing each checkpoint interval's

@@ -62,11 +62,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
+    // wave-uniform wave index: the work item and its tables live in scalar
+    // registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // per-wave LDS ring of the last 4 loaded rows: the centre row of the X
     // update comes from here instead of a second read from memory (as in the
     // forward stage kernel); each lane touches only its own column
     __shared__ double ring_all[4 * 4 * 4 * 64];
-    double* const ring = ring_all + (threadIdx.x >> 6) * (4 * 4 * 64) + lane;
+    double* const ring = ring_all + wave * (4 * 4 * 64) + lane;
     auto ring_put = [&](int e, const double* u) {
         double* q = ring + (e & 3) * (4 * 64);
         q[0] = u[0];
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         u[2] = q[128];
         u[3] = q[192];
     };
-    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
         const nest_prob_state& ps = a.probs[pd.prob];
@@ -187,7 +190,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
             u[3] = p[3 * (long long)s];
         };
 
-        // X window warm-up (as the forward stage kernel)
+        // X window warm-up (as the forward stage kernel; the same flux
+        // arithmetic as the forward kernels of this numerics mode)
         prim4 W1, dW1, FL;
         double Fm[4];
         {
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
             dW1 = sub4(W1, W0);
             prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
             prim4 s0 = slope4<LIM>(d0, dW1);
-            face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+            face_flux<true, true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
             FL = face_plus(W0, s0);
         }
         double Un[4];
@@ -221,7 +225,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
             prim4 dW2 = sub4(W2, W1);
             prim4 s1 = slope4<LIM>(dW1, dW2);
             double Fp[4];
-            face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
+            face_flux<true, true>(FL, face_minus(W1, s1), a.eos, Fp);
             FL = face_plus(W1, s1);
             v[0] = Uc[0] - dtdx * (Fp[0] - Fm[0]);
             v[1] = Uc[1] - dtdx * (Fp[1] - Fm[1]);
@@ -265,7 +269,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
-    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
         const nest_item w = a.items[it];
         const nest_patch_desc pd = a.patches[w.patch];
         const nest_prob_state& ps = a.probs[pd.prob];

@@ -299,6 +299,35 @@ __device__ __forceinline__ double lane_down_z(double x) {
 }
 #endif
 
+#if defined(__HIPCC__)
+// ---------------------------------------------------------------------------
+// global -> LDS row fetches (global_load_lds_dword) and the waits they need
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// s_waitcnt (gfx9 encoding: vmcnt[3:0] in bits 3:0 and vmcnt[5:4] in 15:14,
+// expcnt in 6:4, lgkmcnt in 11:8). The signal fences keep the compiler from
+// moving memory operations across the wait.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void wait_lgkmcnt0() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void compiler_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+__device__ __forceinline__ const char* shfl_cptr(const void* p, int src) {
+    return reinterpret_cast<const char*>(__shfl(reinterpret_cast<long long>(p), src));
+}
+
+#endif
+
 // slope of all 4 quantities from the left and right differences
 template <int LIM>
 __device__ __forceinline__ prim4 slope4(const prim4& dl, const prim4& dr) {

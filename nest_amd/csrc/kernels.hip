@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             dW1 = sub4(W1, W0);
             prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
             prim4 s0 = slope4<LIM>(d0, dW1);
-            face_flux<true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+            face_flux<true, true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
             FL = face_plus(W0, s0);
         }
 
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             prim4 dW2 = sub4(W2, W1);
             prim4 s1 = slope4<LIM>(dW1, dW2);
             double Fp[4];
-            face_flux<true>(FL, face_minus(W1, s1), a.eos, Fp);
+            face_flux<true, true>(FL, face_minus(W1, s1), a.eos, Fp);
             FL = face_plus(W1, s1);
             ux[0] = Uc[0] - dtdx * (Fp[0] - Fm[0]);
             ux[1] = Uc[1] - dtdx * (Fp[1] - Fm[1]);
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             prim4 sy = slope4<LIM>(dl, dr);
             prim4 Rf = shfl_down4(face_minus(Y, sy));
             double G[4];
-            face_flux<true>(face_plus(Y, sy), Rf, a.eos, G);
+            face_flux<true, true>(face_plus(Y, sy), Rf, a.eos, G);
             double Gm[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) Gm[k] = lane_up(G[k]);
@@ -380,29 +380,6 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
 // one slot for the U^n row (2 KB). Dynamic shared memory: waves per block x
 // that (4 waves of stage 2 per SIMD: 16 x 10 KB = the CU's 160 KB).
 // ---------------------------------------------------------------------------
-typedef __attribute__((address_space(3))) void* lds_void_ptr;
-
-// s_waitcnt (gfx9 encoding: vmcnt[3:0] in bits 3:0 and vmcnt[5:4] in 15:14,
-// expcnt in 6:4, lgkmcnt in 11:8). The signal fences keep the compiler from
-// moving memory operations across the wait.
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-__device__ __forceinline__ void wait_lgkmcnt0() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-__device__ __forceinline__ void compiler_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
-
-__device__ __forceinline__ const char* shfl_cptr(const void* p, int src) {
-    return reinterpret_cast<const char*>(__shfl(reinterpret_cast<long long>(p), src));
-}
-
 constexpr int kRowSlot = 4 * 64;  // doubles per ring slot (4 variables x 64 columns)
 constexpr int kWaveLds1 = 4 * kRowSlot;                 // stage 1: 4 row slots
 constexpr int kWaveLds2 = 4 * kRowSlot + kRowSlot;  // stage 2: + 1 U^n slot
@@ -880,7 +857,7 @@ template <int STAGE>
 static void stage2d_dispatch(int lim, const stage_args& a, int grid, int block, hipStream_t s) {
 #if NEST_STAGE_DMA
     // the LDS-ring kernel addresses the state buffers with 32-bit byte offsets
-    if (a.total > 0 && a.total * (long long)sizeof(double) < (1LL << 32)) {
+    if (a.kernel != 1 && a.total > 0 && a.total * (long long)sizeof(double) < (1LL << 32)) {
     const size_t lds = (size_t)(block / 64) * (STAGE == 1 ? kWaveLds1 : kWaveLds2) * sizeof(double);
     switch (lim) {
         case LIM_MC: hipLaunchKernelGGL((stage2d_dma_kernel<LIM_MC, STAGE>), grid, block, lds, s, a); break;

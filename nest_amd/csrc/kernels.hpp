@@ -43,6 +43,7 @@ struct stage_args {
     double* Uo;  // stage 2 output (null: in place of U)
     long long total;  // doubles in each state buffer (all share the patch layout)
     unsigned long long* slots;  // stage 2: per-problem wave-speed slots (NEST_SPEED_SLOTS each)
+    int kernel;  // stage kernel: 0 automatic, 1 register path, 2 LDS-ring path (state below 4 GiB)
 };
 
 // arguments of the two adjoint kernels of one split stage S = Y o X (the

@@ -154,6 +154,7 @@ struct nest_sim2d {
     int gwpb[2] = {4, 4};       // waves per block, stage 1 / stage 2
     int persistent = 0;
     int resident_waves = 0;
+    int stage_kernel = 0;  // 0 automatic, 1 register path, 2 LDS-ring path
 
     // ---- discrete adjoint of fixed-dt rollouts
     std::vector<nest_item> aitems_h[2];  // adjoint strips: Y kernel (58 owned columns), X kernel (64)
@@ -175,6 +176,7 @@ struct nest_sim2d {
     hipEvent_t ev_start = nullptr;
     std::vector<hipEvent_t> ev_rec_c, ev_rev_c;  // per interval: recompute done, reverse done
     int roll_K = 0, roll_m = 0;
+    bool roll_kept = false;  // the last interval's states are in its buffer set
     double roll_dt = 0.0;
 
     ~nest_sim2d() { release(); }
@@ -211,6 +213,7 @@ struct nest_sim2d {
         adj_U1 = adj_Vb = adj_Ub1 = nullptr;
         goff_d = nullptr;
         roll_K = 0;
+        roll_kept = false;
         U_own = U1_own = U = U1 = nullptr;
         patches_d = nullptr;
         gitems_d[0] = gitems_d[1] = nullptr;
@@ -244,6 +247,7 @@ struct nest_sim2d {
         a.geoms = geoms_d;
         a.itab = itab_d;
         a.total = (long long)total;
+        a.kernel = stage_kernel;
         return a;
     }
     int grid_for(int nitems, int waves) const {
@@ -624,6 +628,16 @@ int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int 
     return s->build_items();
 }
 
+int nest_hip_sim2d_set_stage_kernel(nest_sim2d* s, int kind) {
+    if (int rc = valid_sim(s)) return rc;
+    if (kind < 0 || kind > 2) return fail(NEST_ERR_INVALID, "kind: 0 automatic, 1 register path, 2 LDS-ring path");
+    if (kind == 2 && (long long)s->total * (long long)sizeof(double) >= (1LL << 32))
+        return fail(NEST_ERR_UNSUPPORTED, "the LDS-ring stage kernel needs state buffers below 4 GiB");
+    s->stage_kernel = kind;
+    s->drop_graph();
+    return NEST_OK;
+}
+
 int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int waves_per_block) {
     if (int rc = valid_sim(s)) return rc;
     if ((stage != 1 && stage != 2) || rows < 0 || waves_per_block < 0 || waves_per_block > 4)
@@ -935,17 +949,38 @@ int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
     const int m = interval > 0 ? interval : std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
     const int nck = (nsteps + m - 1) / m;
     if (int rc = s->grow(s->ckpt, (size_t)nck)) return rc;
+    // With two or more intervals, the states of the last interval (and their
+    // stage-1 states) are written straight into the buffer set its reverse
+    // reads, so the reverse sweep starts without recomputing it.
+    const bool keep = nck >= 2;
+    const int nsets = std::min(NEST_ADJ_SETS, nck);
+    const int cl = nck - 1, setl = cl % nsets, lenl = nsteps - cl * m;
+    if (keep) {
+        if (int rc = s->grow(s->seg[setl], (size_t)std::max(0, m - 1))) return rc;
+        if (int rc = s->grow(s->seg1[setl], (size_t)m)) return rc;
+    }
     // checkpoint c = state of step c*m: the initial state is copied; every
     // later one is written by the stage 2 before it (out of place) and read
     // from there by the next step, whose stage 2 writes U again
     NEST_HIP_TRY(hipMemcpyAsync(s->ckpt[0], s->U, s->total * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     for (int k = 0; k < nsteps; ++k) {
+        if (keep && k >= cl * m) {
+            // last interval: state i in the set (the checkpoint first), the
+            // final state into U
+            const int i = k - cl * m;
+            double* src = i == 0 ? s->ckpt[cl] : s->seg[setl][i - 1];
+            double* dst = i + 1 < lenl ? s->seg[setl][i] : s->U;
+            s->stage_on(1, src, s->seg1[setl][i], dt);
+            s->stage_on(2, src, s->seg1[setl][i], dt, dst);
+            continue;
+        }
         double* src = (k % m == 0 && k > 0) ? s->ckpt[k / m] : s->U;
         double* dst = ((k + 1) % m == 0 && k + 1 < nsteps) ? s->ckpt[(k + 1) / m] : s->U;
         s->stage_on(1, src, s->U1, dt);
         s->stage_on(2, src, s->U1, dt, dst == src ? nullptr : dst);
     }
     s->fill(s->U);
+    s->roll_kept = keep;
     s->roll_K = nsteps;
     s->roll_m = m;
     s->roll_dt = dt;
@@ -991,13 +1026,16 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     // everything enqueued before (the rollout and its checkpoints) comes first
     NEST_HIP_TRY(hipEventRecord(s->ev_start, s->stream));
     NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_start, 0));
-    for (int c = nck - 1; c >= std::max(0, nck - (nsets - 1)); --c) NEST_HIP_TRY(recompute(c));
+    // the last interval's states were kept by the rollout (with 2+ intervals)
+    const bool kept = s->roll_kept && nck >= 2;
+    for (int c = nck - 1; c >= std::max(0, nck - (nsets - 1)); --c)
+        if (!(kept && c == nck - 1)) NEST_HIP_TRY(recompute(c));
     // reverse of interval c (handle's stream), then the recompute S - 1
     // intervals further down (auxiliary stream; it waits for the reverse
     // that last read its buffer set)
     for (int c = nck - 1; c >= 0; --c) {
         const int set = c % nsets, len = std::min(m, K - c * m);
-        NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
+        if (!(kept && c == nck - 1)) NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
         for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
         NEST_HIP_TRY(hipEventRecord(s->ev_rev_c[c], s->stream));
         if (c - (nsets - 1) >= 0) NEST_HIP_TRY(recompute(c - (nsets - 1)));

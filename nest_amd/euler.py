@@ -198,6 +198,10 @@ class Sim2D:
         check(self.L.nest_hip_sim2d_info(self.h, C.byref(total), C.byref(npatch), C.byref(nprob), C.byref(nitems)))
         self.nitems = nitems.value
 
+    def set_stage_kernel(self, kind: str = "auto") -> None:
+        """Which 2D stage kernel runs: "auto", "register" or "lds_ring"."""
+        check(self.L.nest_hip_sim2d_set_stage_kernel(self.h, {"auto": 0, "register": 1, "lds_ring": 2}[kind]))
+
     def set_stage_tiling(self, stage: int, rows: int = 0, waves_per_block: int = 0) -> None:
         check(self.L.nest_hip_sim2d_set_stage_tiling(self.h, stage, rows, waves_per_block))
 

@@ -140,13 +140,20 @@ def _state(kind, nx, ny):
     return np.ascontiguousarray(G)
 
 
+# "auto" runs the LDS-ring stage kernel at these sizes; "register" forces the
+# register-path kernel (used for state buffers of 4 GiB and more)
+STAGE_KERNELS = ["auto", "register"]
+
+
 @pytest.mark.parametrize("case", CASES_2D)
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-def test_2d_mesh_lockstep(oracle, case, mode):
+@pytest.mark.parametrize("kernel", STAGE_KERNELS)
+def test_2d_mesh_lockstep(oracle, case, mode, kernel):
     nx, ny, npx, npy, bcx, bcy, kind = case
-    print_case("2d lockstep", case, mode)
+    print_case("2d lockstep", case, mode, kernel)
     dx, dy = 1.0 / nx, 1.0 / ny
     sim = Sim2D.mesh(nx, ny, npx, npy, dx, dy, bcx=bcx, bcy=bcy, mode=mode)
+    sim.set_stage_kernel(kernel)
     m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
     U = sim.scatter(_state(kind, nx, ny))
     oracle.mesh_fill(U, m)
@@ -187,12 +194,14 @@ def test_2d_halo_fill_bitwise(oracle):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-def test_2d_device_dt_run(oracle, graph):
+@pytest.mark.parametrize("kernel", STAGE_KERNELS)
+def test_2d_device_dt_run(oracle, graph, kernel):
     """Device-resident dt (reduction fused into stage 2 + finalize kernel):
     dt sequence and states equal the oracle's compute_dt_2d loop."""
     nx, ny, npx, npy = 64, 48, 2, 2
-    print_case("2d device dt", graph)
+    print_case("2d device dt", graph, kernel)
     sim = Sim2D.mesh(nx, ny, npx, npy)
+    sim.set_stage_kernel(kernel)
     m = MeshSpec(nx, ny, npx, npy)
     U = sim.scatter(P.sod2d(nx, ny))
     oracle.mesh_fill(U, m)
@@ -207,12 +216,14 @@ def test_2d_device_dt_run(oracle, graph):
     assert np.array_equal(sim.gather(sim.download()), sim.gather(ref))
 
 
-def test_2d_limiters(oracle):
+@pytest.mark.parametrize("kernel", STAGE_KERNELS)
+def test_2d_limiters(oracle, kernel):
     nx, ny = 48, 40
     for lim in ["mc", "none"]:
         for mode in ["exact", "fast"]:
-            print_case("limiter", lim, mode)
+            print_case("limiter", lim, mode, kernel)
             sim = Sim2D.mesh(nx, ny, 2, 2, limiter=lim, mode=mode, bcx="periodic", bcy="periodic")
+            sim.set_stage_kernel(kernel)
             m = MeshSpec(nx, ny, 2, 2, bcx="periodic", bcy="periodic")
             U = sim.scatter(P.smooth_periodic(48)[:, :nx, :ny].copy())
             oracle.mesh_fill(U, m)
