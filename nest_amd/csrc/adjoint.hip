@@ -29,9 +29,11 @@
 #include "kernels.hpp"
 
 // Minimum waves per SIMD requested for the Y and X kernels (register budget
-// 512/N per lane).
+// 512/N per lane). With 32-bit element offsets (buffers below 2^32
+// elements, checked by the host) the fast Y kernel fits 3 waves (168
+// VGPRs) without spills; the X kernel needs 2.
 #ifndef NEST_ADJY_WAVES
-#define NEST_ADJY_WAVES 1
+#define NEST_ADJY_WAVES (NEST_FAST ? 3 : 2)  // exact numerics' IEEE divisions need more registers
 #endif
 #ifndef NEST_ADJX_WAVES
 #define NEST_ADJX_WAVES 2
@@ -45,12 +47,16 @@ __device__ __forceinline__ prim4 shup4(const prim4& a, int d) {
 __device__ __forceinline__ prim4 shdn4(const prim4& a, int d) {
     return {__shfl_down(a.r, d), __shfl_down(a.u, d), __shfl_down(a.p, d), __shfl_down(a.q, d)};
 }
-// delta 1 by DPP wave shifts (same lane semantics as the shuffles)
+// delta 1 by zero-fill DPP wave shifts: lane 0 (up1) / lane 63 (dn1)
+// receive 0. In the Y-sweep reverse those values reach only lane 0's and
+// lanes 62-63's own faces, which are zeroed before use (has_face), and the
+// limiter reverse of lanes 0 and 63, whose slope cotangent is then 0; owned
+// lanes 3..60 and the outflow fold (which reads lanes >= 1) never see them.
 __device__ __forceinline__ prim4 up1(const prim4& a) {
-    return {lane_up(a.r), lane_up(a.u), lane_up(a.p), lane_up(a.q)};
+    return {lane_up_z(a.r), lane_up_z(a.u), lane_up_z(a.p), lane_up_z(a.q)};
 }
 __device__ __forceinline__ prim4 dn1(const prim4& a) {
-    return {lane_down(a.r), lane_down(a.u), lane_down(a.p), lane_down(a.q)};
+    return {lane_down_z(a.r), lane_down_z(a.u), lane_down_z(a.p), lane_down_z(a.q)};
 }
 
 // ---------------------------------------------------------------------------
@@ -99,7 +105,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         const nest_patch_desc qd = a.patches[q];
         const int snye = qd.ny + 4;
         const int svs = (qd.nx + 4) * snye;
-        const long long col = qd.off + lc;  // this lane's column, extended row 0
+        // this lane's column at extended row 0, as an element offset (a state
+        // buffer stays far below 2^32 elements), and its row offset within
+        const unsigned col = (unsigned)(qd.off + lc);
         const bool ring = G.bcy == 0;
         const bool real = ring || (gcol >= 0 && gcol < G.NY);  // outflow ghosts are not outputs
         const bool own = lane >= 3 && lane < 61 && gcol < G.NY;
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
 
         // interior row i of buffer B at this lane's column
         auto load_int = [&](const double* B, int i, double* u) {
-            const double* p = B + col + (long long)(i + 2) * snye;
+            const double* p = B + (col + (unsigned)(i + 2) * (unsigned)snye);
             u[0] = p[0];
             u[1] = p[svs];
             u[2] = p[2 * (long long)svs];
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
             // rho <- G0, m_u <- G3, m_v <- G1, E <- G2)
             double An[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) An[k] = lane_down(A[k]);
+            for (int k = 0; k < 4; ++k) An[k] = lane_down_z(A[k]);
             double Fb[4];
             Fb[0] = dtdy * (An[0] - A[0]);
             Fb[1] = dtdy * (An[2] - A[2]);
@@ -162,7 +170,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
             double Ub[4];
             prim2d_rev(Y, false, Wb, a.eos, Ub);
             if (own) {
-                double* po = a.Vb + col + (long long)(i + 2) * snye;
+                double* po = a.Vb + (col + (unsigned)(i + 2) * (unsigned)snye);
                 po[0] = A[0] + Ub[0];
                 po[svs] = A[1] + Ub[1];
                 po[2 * (long long)svs] = A[2] + Ub[2];
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
                 s = (qd2.nx + 4) * qnye;
                 p = a.U + qd2.off + (long long)le2 * qnye + lc2;
             } else {
-                p = a.U + col + (long long)e * snye;
+                p = a.U + (col + (unsigned)e * (unsigned)snye);
                 s = svs;
             }
             u[0] = p[0];
@@ -284,7 +292,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
         const nest_patch_desc qd = a.patches[q];
         const int snye = qd.ny + 4;
         const int svs = (qd.nx + 4) * snye;
-        const long long col = qd.off + lc;
+        const unsigned col = (unsigned)(qd.off + lc);  // element offset (buffers < 2^32 elements)
         const bool ring = G.bcx == 0;
         const bool lo = OUTX && !ring && pd.x0 + w.r0 == 0;     // outflow end at row 0
         const bool hi = OUTX && !ring && pd.x0 + w.r1 == G.NX;  // outflow end at row NX-1
@@ -303,7 +311,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
                 s = (qd2.nx + 4) * qnye;
                 p = B + qd2.off + (long long)le2 * qnye + lc2;
             } else {
-                p = B + col + (long long)e * snye;
+                p = B + (col + (unsigned)e * (unsigned)snye);
                 s = svs;
             }
             u[0] = p[0];
@@ -321,17 +329,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
         auto emit = [&](int row, const prim4& Wp, const prim4& Wbar) {
             double Ub[4];
             prim2d_rev(Wp, true, Wbar, a.eos, Ub);
-            const long long o = col + (long long)(row + 2) * snye;
+            const unsigned o = col + (unsigned)(row + 2) * (unsigned)snye;
             double r[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = a.Vb[o + k * (long long)svs] + Ub[k];
+            for (int k = 0; k < 4; ++k) r[k] = a.Vb[o + (unsigned)k * (unsigned)svs] + Ub[k];
             if (a.H) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) r[k] += a.h * a.H[o + k * (long long)svs];
+                for (int k = 0; k < 4; ++k) r[k] += a.h * a.H[o + (unsigned)k * (unsigned)svs];
             }
             if (own) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) a.out[o + k * (long long)svs] = r[k];
+                for (int k = 0; k < 4; ++k) a.out[o + (unsigned)k * (unsigned)svs] = r[k];
             }
         };
 

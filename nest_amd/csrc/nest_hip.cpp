@@ -406,6 +406,9 @@ struct nest_sim2d {
         return NEST_OK;
     }
     int ensure_adjoint_scratch() {
+        // the adjoint kernels address every buffer with 32-bit element offsets
+        if ((unsigned long long)total >= (1ULL << 32))
+            return fail(NEST_ERR_UNSUPPORTED, "adjoint: state buffers of 2^32 elements or more");
         for (double** p : {&adj_U1, &adj_Vb, &adj_Ub1})
             if (!*p) {
                 if (int rc = dev_alloc(p, total)) return rc;
