@@ -29,11 +29,15 @@
 #include "kernels.hpp"
 
 // Minimum waves per SIMD requested for the Y and X kernels (register budget
-// 512/N per lane). With 32-bit element offsets (buffers below 2^32
-// elements, checked by the host) the fast Y kernel fits 3 waves (168
-// VGPRs) without spills; the X kernel needs 2.
+// 512/N per lane). Buffers are addressed by 32-bit element offsets (below
+// 2^32 elements, checked by the host), which keeps the register counts down.
+// Y kernel: 3 waves per SIMD in fast numerics with the minmod / none
+// limiters (166 VGPRs); exact numerics (IEEE divisions) and the MC limiter
+// (coefficient state) need more registers and stay at 2. X kernel: 2 (its
+// interior runs two march positions per trip; one position per trip fits 3
+// waves in the periodic fast instances but measured 4% slower overall).
 #ifndef NEST_ADJY_WAVES
-#define NEST_ADJY_WAVES (NEST_FAST ? 3 : 2)  // exact numerics' IEEE divisions need more registers
+#define NEST_ADJY_WAVES(LIM) (NEST_FAST && (LIM) != LIM_MC ? 3 : 2)
 #endif
 #ifndef NEST_ADJX_WAVES
 #define NEST_ADJX_WAVES 2
@@ -63,7 +67,7 @@ __device__ __forceinline__ prim4 dn1(const prim4& a) {
 // Y kernel
 // ---------------------------------------------------------------------------
 template <int LIM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_WAVES, 8))) void adj_y_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_WAVES(LIM), 8))) void adj_y_kernel(
     adj_args a) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
