@@ -91,7 +91,10 @@ class context {
             boundary_t bcx = boundary_t::periodic, boundary_t bcy = boundary_t::periodic, int limiter = 0,
             numerics mode = numerics::exact)
         : shape_(infer_mesh(patches)) {
-        auto bc = [](boundary_t b) { return b == boundary_t::outflow ? NEST_BC_OUTFLOW : NEST_BC_PERIODIC; };
+        auto bc = [](boundary_t b) {
+            return b == boundary_t::outflow ? NEST_BC_OUTFLOW
+                                            : (b == boundary_t::reflecting ? NEST_BC_REFLECTING : NEST_BC_PERIODIC);
+        };
         check(nest_hip_mesh2d_create(shape_.nx, shape_.ny, shape_.npx, shape_.npy, dx, dy, gamma, bc(bcx), bc(bcy),
                                      limiter, static_cast<int>(mode), &sim_),
               "mesh2d_create");

@@ -3,15 +3,20 @@
 // API-compatible with the reference's include/nest/patch.hpp: halo_region_t /
 // halo_exchange_plan_t (patch.hpp:14-37), patch_t (:43-81), the periodic mesh
 // builders (:88-147, :207-326) and the plan executors (:153-204). Own
-// implementation, plus two extensions the Euler step needs:
+// implementation, plus three extensions the Euler step needs:
 //   * physical boundaries: a region with peer_index < 0 is skipped, as in the
-//     reference, unless its `bc` is boundary_t::outflow; then its source is
-//     this patch's own boundary cells, and a source extent of 1 along a
-//     dimension is broadcast over the destination (zero-gradient ghosts);
+//     reference, unless its `bc` is boundary_t::outflow or ::reflecting; then
+//     its source is this patch's own boundary cells, and a source extent of 1
+//     along a dimension is broadcast over the destination (zero-gradient
+//     ghosts);
+//   * mirrored regions (reflecting walls): bit k of `mirror` walks the source
+//     backwards along dimension k, bit v of `negate` flips the sign of
+//     variable v (the wall-normal momentum);
 //   * build_mesh_2d(): a 2D tensor-product mesh whose plans cover faces AND
-//     corners, with periodic or outflow boundaries per dimension. Corners
-//     take the value of the global cell reached by mapping each coordinate
-//     through its boundary condition — the halo a fused X-then-Y stage needs.
+//     corners, with periodic, outflow or reflecting boundaries per dimension.
+//     Corners take the value of the global cell reached by mapping each
+//     coordinate through its boundary condition — the halo a fused X-then-Y
+//     stage needs.
 #pragma once
 #include <algorithm>
 #include <cstring>
@@ -22,7 +27,7 @@
 
 namespace nest {
 
-enum class boundary_t { none, periodic, outflow };
+enum class boundary_t { none, periodic, outflow, reflecting };
 
 template <std::size_t Dim>
 struct halo_region_t {
@@ -31,14 +36,16 @@ struct halo_region_t {
     index_space_t<Dim> dst_space;  // ghost box in this patch
     std::size_t copy_size = 0;     // elements per variable
     boundary_t bc = boundary_t::none;
+    unsigned mirror = 0;           // bit k: source walked backwards along dimension k
+    unsigned negate = 0;           // bit v: variable v changes sign (wall-normal momentum)
 };
 
 template <std::size_t Dim>
 struct halo_exchange_plan_t {
     std::vector<halo_region_t<Dim>> regions;
     void add_region(int peer, index_space_t<Dim> src, index_space_t<Dim> dst,
-                    boundary_t bc = boundary_t::none) {
-        regions.push_back(halo_region_t<Dim>{peer, src, dst, size(dst), bc});
+                    boundary_t bc = boundary_t::none, unsigned mirror = 0, unsigned negate = 0) {
+        regions.push_back(halo_region_t<Dim>{peer, src, dst, size(dst), bc, mirror, negate});
     }
     std::size_t num_regions() const { return regions.size(); }
 };
@@ -72,24 +79,30 @@ void execute_exchange(patch_t<T, Dim>& patch, const std::vector<patch_t<T, Dim>>
         const patch_t<T, Dim>* src;
         if (r.peer_index >= 0) {
             src = &all[static_cast<std::size_t>(r.peer_index)];
-        } else if (r.bc == boundary_t::outflow) {
+        } else if (r.bc == boundary_t::outflow || r.bc == boundary_t::reflecting) {
             src = &patch;
         } else {
             continue;  // physical boundary left to the caller (reference behaviour)
         }
         for (std::size_t v = 0; v < patch.nvars(); ++v) {
-            if (Dim == 1 && r.peer_index >= 0 && size(r.src_space) == r.copy_size) {
+            if (Dim == 1 && r.peer_index >= 0 && size(r.src_space) == r.copy_size && !r.mirror && !r.negate) {
                 std::memcpy(patch.state.conserved.var_data(v) + linear_index(patch.extended, start(r.dst_space)),
                             src->state.conserved.var_data(v) + linear_index(src->extended, start(r.src_space)),
                             r.copy_size * sizeof(T));
                 continue;
             }
+            const bool neg = (r.negate >> v) & 1u;
             for (auto d : r.dst_space) {
                 ivec_t<Dim> s;
-                for (std::size_t k = 0; k < Dim; ++k)
+                for (std::size_t k = 0; k < Dim; ++k) {
+                    const int o = d[k] - r.dst_space._start[k];
                     s[k] = r.src_space._start[k] +
-                           (r.src_space._shape[k] == 1 ? 0 : d[k] - r.dst_space._start[k]);
-                patch(v, d) = (*src)(v, s);
+                           (r.src_space._shape[k] == 1
+                                ? 0
+                                : ((r.mirror >> k) & 1u ? static_cast<int>(r.dst_space._shape[k]) - 1 - o : o));
+                }
+                const T x = (*src)(v, s);
+                patch(v, d) = neg ? -x : x;
             }
         }
     }
@@ -116,12 +129,14 @@ void fill_all_halos(std::vector<patch_t<T, Dim>>& patches) {
 // ---------------------------------------------------------------------------
 namespace detail {
 // ghost strip on `side` (-1 low, +1 high) of partition p along one axis:
-// source partition q and the first source cell along the axis, or an outflow
-// source (q = p, the boundary cell, broadcast)
+// source partition q and the first source cell along the axis, an outflow
+// source (q = p, the boundary cell, broadcast), or a reflecting one (q = p,
+// the `halo` cells next to the wall, walked backwards)
 struct axis_source {
     int q;
     int first;
     bool outflow;
+    bool mirror;
 };
 template <std::size_t Dim>
 inline axis_source ghost_source(int side, int p, const std::vector<index_space_t<Dim>>& parts, int axis, int halo,
@@ -130,15 +145,17 @@ inline axis_source ghost_source(int side, int p, const std::vector<index_space_t
     if (side < 0) {
         if (p > 0 || bc == boundary_t::periodic) {
             int q = (p - 1 + n) % n;
-            return {q, upper(parts[q])[axis] - halo, false};
+            return {q, upper(parts[q])[axis] - halo, false, false};
         }
-        return {p, start(parts[p])[axis], true};
+        if (bc == boundary_t::reflecting) return {p, start(parts[p])[axis], false, true};
+        return {p, start(parts[p])[axis], true, false};
     }
     if (p < n - 1 || bc == boundary_t::periodic) {
         int q = (p + 1) % n;
-        return {q, start(parts[q])[axis], false};
+        return {q, start(parts[q])[axis], false, false};
     }
-    return {p, upper(parts[p])[axis] - 1, true};
+    if (bc == boundary_t::reflecting) return {p, upper(parts[p])[axis] - halo, false, true};
+    return {p, upper(parts[p])[axis] - 1, true, false};
 }
 }  // namespace detail
 
@@ -164,7 +181,10 @@ std::vector<patch_t<T, Dim>> build_periodic_mesh_1d(index_space_t<Dim> domain, i
 
 // 2D tensor-product mesh, patch index = py*npx + px (reference
 // patch.hpp:219-229). corners = false gives the reference's face-only plans
-// (build_periodic_mesh_2d); corners = true adds the 4 corner blocks.
+// (build_periodic_mesh_2d); corners = true adds the 4 corner blocks. A
+// reflecting wall along dimension k negates variable 1 + k, the wall-normal
+// momentum of the (rho, rho u, rho v, E) layout; it needs patches of at least
+// `halo` cells along k.
 template <typename T>
 std::vector<patch_t<T, 2>> build_mesh_2d(index_space_t<2> domain, int npx, int npy, std::size_t nvars, int halo,
                                          boundary_t bcx, boundary_t bcy, bool corners = true) {
@@ -186,6 +206,7 @@ std::vector<patch_t<T, 2>> build_mesh_2d(index_space_t<2> domain, int npx, int n
                     uvec_t<2> dn, sn;
                     int qx = px, qy = py;
                     bool outflow = false;
+                    unsigned mirror = 0, negate = 0;
                     // dim 0
                     if (sx == 0) {
                         d0[0] = s0[0] = start(me.interior)[0];
@@ -198,6 +219,7 @@ std::vector<patch_t<T, 2>> build_mesh_2d(index_space_t<2> domain, int npx, int n
                         s0[0] = a.first;
                         sn[0] = a.outflow ? 1u : static_cast<unsigned>(halo);
                         outflow |= a.outflow;
+                        if (a.mirror) mirror |= 1u, negate |= 2u;
                     }
                     // dim 1
                     if (sy == 0) {
@@ -211,14 +233,19 @@ std::vector<patch_t<T, 2>> build_mesh_2d(index_space_t<2> domain, int npx, int n
                         s0[1] = a.first;
                         sn[1] = a.outflow ? 1u : static_cast<unsigned>(halo);
                         outflow |= a.outflow;
+                        if (a.mirror) mirror |= 2u, negate |= 4u;
                     }
                     const int q = qy * npx + qx;
                     // a region whose source is another patch (or itself through
                     // a periodic wrap) is an exchange; one that reads this
-                    // patch's own boundary cells is an outflow fill
-                    const bool self_bc = outflow && q == me.index;
-                    me.exchange_plan.add_region(self_bc ? -1 : q, index_space(s0, sn), index_space(d0, dn),
-                                                outflow ? boundary_t::outflow : boundary_t::periodic);
+                    // patch's own boundary cells is a wall fill (outflow copy
+                    // or reflecting mirror)
+                    const bool wall = outflow || mirror;
+                    const bool self_bc = wall && q == me.index;
+                    me.exchange_plan.add_region(
+                        self_bc ? -1 : q, index_space(s0, sn), index_space(d0, dn),
+                        mirror ? boundary_t::reflecting : (outflow ? boundary_t::outflow : boundary_t::periodic),
+                        mirror, negate);
                 }
         }
     return patches;

@@ -40,7 +40,12 @@ enum nest_status {
     NEST_ERR_NONFINITE = 4,   /* a problem produced a non-finite dt */
     NEST_ERR_UNSUPPORTED = 5  /* configuration not supported */
 };
-enum nest_bc { NEST_BC_PERIODIC = 0, NEST_BC_OUTFLOW = 1 };
+/* Physical boundaries. PERIODIC and OUTFLOW are the reference's two
+   (periodic wrap, zero-gradient copy of the edge cell). REFLECTING has no
+   counterpart in the reference: a solid wall whose ghost cells mirror the
+   interior across the wall with the wall-normal momentum negated. The forward
+   path supports it; the adjoint entry points reject it (NEST_ERR_UNSUPPORTED). */
+enum nest_bc { NEST_BC_PERIODIC = 0, NEST_BC_OUTFLOW = 1, NEST_BC_REFLECTING = 2 };
 /* limiter_t of reference examples/sod1d/euler1d.hpp:106-110 */
 enum nest_limiter { NEST_LIM_MINMOD = 0, NEST_LIM_MC = 1, NEST_LIM_NONE = 2 };
 enum nest_mode { NEST_MODE_EXACT = 0, NEST_MODE_FAST = 1 };
@@ -63,7 +68,7 @@ int nest_hip_device_copy(const double* src, double* dst, long long n, void* stre
 /* ===================================================================== */
 typedef struct nest_solver1d nest_solver1d;
 
-/* n interior cells, halo 2, cell size dx, gamma, bc (periodic/outflow),
+/* n interior cells, halo 2, cell size dx, gamma, bc (periodic/outflow/reflecting),
    limiter, mode. Replaces euler1d_gpu_create (euler1d_solver.hpp:27). */
 int nest_hip_solver1d_create(int n, double dx, double gamma, int bc, int limiter, int mode,
                              nest_solver1d** out);
@@ -152,7 +157,7 @@ int nest_hip_sim2d_download(nest_sim2d* s, double* host);
 int nest_hip_sim2d_upload_patch(nest_sim2d* s, int patch, const double* host);
 int nest_hip_sim2d_download_patch(nest_sim2d* s, int patch, double* host);
 /* Halo exchange of U: every ghost cell (faces and corners) of every patch
-   from its neighbour or physical boundary (periodic wrap / outflow copy);
+   from its neighbour or physical boundary (periodic wrap / outflow copy / wall mirror);
    one launch. Extends execute_all_exchanges (reference patch.hpp:153-190). */
 int nest_hip_sim2d_fill_halos(nest_sim2d* s);
 /* The same exchange on a chosen buffer: 0 = U, 1 = U1 (the first-stage

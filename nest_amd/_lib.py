@@ -16,7 +16,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "nest_hip.h")
 
 NEST_OK = 0
 STATUS = {0: "OK", 1: "INVALID", 2: "ALLOC", 3: "HIP", 4: "NONFINITE", 5: "UNSUPPORTED"}
-BC = {"periodic": 0, "outflow": 1}
+BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 LIMITER = {"minmod": 0, "mc": 1, "none": 2}
 MODE = {"exact": 0, "fast": 1}
 

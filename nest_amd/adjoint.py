@@ -23,6 +23,14 @@ import torch
 from .euler import Sim2D
 
 
+def _check_bc(bc: str) -> None:
+    """The reverse sweep differentiates periodic and outflow boundaries; the
+    library rejects a reflecting wall too (NEST_ERR_UNSUPPORTED), this says so
+    before any device work."""
+    if bc not in ("periodic", "outflow"):
+        raise ValueError(f"adjoint rollouts support bc 'periodic' or 'outflow', not {bc!r}")
+
+
 class Rollout:
     """K fixed-dt RK2 steps as a differentiable map U0 -> U_K."""
 
@@ -42,6 +50,7 @@ class Rollout:
     @classmethod
     def mesh(cls, nx, ny, npx=1, npy=1, nsteps=1, dt=1e-3, bc="periodic", limiter="minmod", mode="fast",
              interval=0, gamma=1.4, tune=False):
+        _check_bc(bc)
         sim = Sim2D.mesh(nx, ny, npx, npy, gamma=gamma, bcx=bc, bcy=bc, limiter=limiter, mode=mode)
         ro = cls(sim, (4, nx, ny), nsteps, dt, interval)
         if tune:
@@ -51,6 +60,7 @@ class Rollout:
     @classmethod
     def batch(cls, nprob, nx, ny, nsteps=1, dt=1e-3, bc="periodic", limiter="minmod", mode="fast", interval=0,
               gamma=1.4, tune=False):
+        _check_bc(bc)
         sim = Sim2D.batch([(nx, ny)] * nprob, gamma=gamma, bc=bc, limiter=limiter, mode=mode)
         ro = cls(sim, (nprob, 4, nx, ny), nsteps, dt, interval)
         if tune:

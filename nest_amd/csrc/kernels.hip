@@ -54,6 +54,12 @@ namespace NEST_NS {
 
 __device__ __forceinline__ double ld(const double* p) { return *p; }
 
+// x with its sign bit XORed with the sign bit of `mask` (mask 0: x itself;
+// 0x80000000: -x, bit for bit)
+__device__ __forceinline__ double xor_sign(double x, int mask) {
+    return __hiloint2double(__double2hiint(x) ^ mask, __double2loint(x));
+}
+
 __device__ __forceinline__ prim4 shfl_up4(const prim4& a) {
     return {lane_up(a.r), lane_up(a.u), lane_up(a.p), lane_up(a.q)};
 }
@@ -183,6 +189,11 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
             u[1] = ld(p + s);
             u[2] = ld(p + 2 * (size_t)s);
             u[3] = ld(p + 3 * (size_t)s);
+            if (a.gather) {  // past a reflecting wall: the normal momentum is negated
+                const int fl = nest_gather_flip(pd, G, e, g);
+                if (fl & 2) u[1] = -u[1];
+                if (fl & 4) u[2] = -u[2];
+            }
         };
 
         // ---- X window warm-up: rows i = r0-2 .. r0+1 (extended rows r0 .. r0+3).
@@ -496,11 +507,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                                  4, 0, 0);
             }
         };
+        // past a reflecting wall the normal momentum is negated (gather
+        // mode; per-patch strips read ghost cells the plan filled): rho v on
+        // lanes past a y wall, rho u on rows past an x wall
+        // (negation is a flip of the sign bit: XOR of the high word with a
+        // mask that is 0 where nothing is negated)
+        const int sign_v = a.gather && (nest_gather_flip(pd, G, 2, g) & 4) ? (int)0x80000000 : 0;
         auto read_row = [&](int e, double* u) {
             const double* q = ring + (e & 3) * kRowSlot + lane;
+            const int sign_u = a.gather && G.bcx == 2 && (nest_gather_flip(pd, G, e, g) & 2) ? (int)0x80000000 : 0;
             u[0] = q[0];
-            u[1] = q[64];
-            u[2] = q[128];
+            u[1] = xor_sign(q[64], sign_u);
+            u[2] = xor_sign(q[128], sign_v);
             u[3] = q[192];
         };
 
@@ -679,10 +697,11 @@ __global__ __launch_bounds__(256) void fill_regions_kernel(double* base, const n
             const int c = t - k * n;
             const int ia = c / g.nj;
             const int jb = c - ia * g.nj;
-            const int si = g.si + (g.bi ? 0 : ia);
-            const int sj = g.sj + (g.bj ? 0 : jb);
+            const int si = g.si + (g.bi ? 0 : (g.mi ? g.ni - 1 - ia : ia));
+            const int sj = g.sj + (g.bj ? 0 : (g.mj ? g.nj - 1 - jb : jb));
+            const double v = base[g.src_off + (long long)k * g.src_vs + (long long)si * g.src_nye + sj];
             base[g.dst_off + (long long)k * g.dst_vs + (long long)(g.di + ia) * g.dst_nye + g.dj + jb] =
-                base[g.src_off + (long long)k * g.src_vs + (long long)si * g.src_nye + sj];
+                (g.neg >> k) & 1 ? -v : v;
         }
     }
 }
@@ -830,7 +849,8 @@ __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
     if ((threadIdx.x & 63) == 0) atomicMax(&a.ps->smax[0], sm);
 }
 
-// periodic (benchmark_euler1d.cpp:102-110) or outflow ghost fill, 3 variables
+// periodic (benchmark_euler1d.cpp:102-110), outflow or reflecting (mirror
+// image, momentum negated) ghost fill, 3 variables
 __global__ void fill1d_kernel(double* U, int n, int bc, const nest_prob_state* ps) {
     if (ps && ps->done) return;
     const int t = threadIdx.x;  // 3 vars x 2 ghosts x 2 sides = 12 threads
@@ -841,6 +861,11 @@ __global__ void fill1d_kernel(double* U, int n, int bc, const nest_prob_state* p
     if (bc == 0) {
         if (side == 0) u[g] = u[n + g];
         else u[2 + n + g] = u[2 + g];
+    } else if (bc == 2) {
+        const double v = side == 0 ? u[3 - g] : u[n + 1 - g];
+        const double r = k == 1 ? -v : v;
+        if (side == 0) u[g] = r;
+        else u[2 + n + g] = r;
     } else {
         if (side == 0) u[g] = u[2];
         else u[2 + n + g] = u[n + 1];

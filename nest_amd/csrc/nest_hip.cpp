@@ -409,6 +409,10 @@ struct nest_sim2d {
         // the adjoint kernels address every buffer with 32-bit element offsets
         if ((unsigned long long)total >= (1ULL << 32))
             return fail(NEST_ERR_UNSUPPORTED, "adjoint: state buffers of 2^32 elements or more");
+        // their halo transpose folds periodic and outflow ghosts only
+        for (const auto& G : tb.geoms)
+            if (G.bcx == 2 || G.bcy == 2)
+                return fail(NEST_ERR_UNSUPPORTED, "adjoint: reflecting boundaries are not differentiated");
         for (double** p : {&adj_U1, &adj_Vb, &adj_Ub1})
             if (!*p) {
                 if (int rc = dev_alloc(p, total)) return rc;
@@ -523,7 +527,8 @@ int nest_hip_mesh2d_create(int nx, int ny, int npx, int npy, double dx, double d
     if (int rc = check_limiter_mode(limiter, mode)) return rc;
     if (nx < 2 || ny < 2 || npx < 1 || npy < 1 || nx / npx < 2 || ny / npy < 2)
         return fail(NEST_ERR_INVALID, "every patch needs at least 2 interior cells per dimension");
-    if ((bcx != 0 && bcx != 1) || (bcy != 0 && bcy != 1)) return fail(NEST_ERR_INVALID, "bc must be 0 or 1");
+    if (bcx < 0 || bcx > 2 || bcy < 0 || bcy > 2)
+        return fail(NEST_ERR_INVALID, "bc must be 0 (periodic), 1 (outflow) or 2 (reflecting)");
     if (!(dx > 0.0) || !(dy > 0.0) || !(gamma > 1.0)) return fail(NEST_ERR_INVALID, "need dx, dy > 0, gamma > 1");
     if ((double)(nx / npx + 5) * (ny / npy + 5) > 2.0e9) return fail(NEST_ERR_UNSUPPORTED, "patch too large");
     auto* s = new nest_sim2d();
@@ -548,7 +553,7 @@ int nest_hip_batch2d_create(int nprob, const int* nx, const int* ny, const doubl
     *out = nullptr;
     if (int rc = check_limiter_mode(limiter, mode)) return rc;
     if (nprob < 1 || !nx || !ny || !dx || !dy) return fail(NEST_ERR_INVALID, "bad batch description");
-    if (bc != 0 && bc != 1) return fail(NEST_ERR_INVALID, "bc must be 0 or 1");
+    if (bc < 0 || bc > 2) return fail(NEST_ERR_INVALID, "bc must be 0 (periodic), 1 (outflow) or 2 (reflecting)");
     auto* s = new nest_sim2d();
     s->mode = mode;
     s->lim = limiter;
@@ -1093,8 +1098,8 @@ int nest_hip_solver1d_create(int n, double dx, double gamma, int bc, int limiter
     if (!out) return fail(NEST_ERR_INVALID, "out is null");
     *out = nullptr;
     if (int rc = check_limiter_mode(limiter, mode)) return rc;
-    if (n < 2 || !(dx > 0.0) || !(gamma > 1.0) || (bc != 0 && bc != 1))
-        return fail(NEST_ERR_INVALID, "need n >= 2, dx > 0, gamma > 1, bc in {0,1}");
+    if (n < 2 || !(dx > 0.0) || !(gamma > 1.0) || bc < 0 || bc > 2)
+        return fail(NEST_ERR_INVALID, "need n >= 2, dx > 0, gamma > 1, bc in {0,1,2}");
     auto* s = new nest_solver1d();
     s->n = n;
     s->bc = bc;

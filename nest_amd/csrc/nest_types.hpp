@@ -29,7 +29,7 @@ struct nest_patch_desc {
 struct nest_prob_geom {
     int patch_base, npx, npy;
     int NX, NY;
-    int bcx, bcy;  // 0 periodic, 1 outflow (zero-gradient)
+    int bcx, bcy;  // 0 periodic, 1 outflow (zero-gradient), 2 reflecting wall
     int tab;
 };
 
@@ -63,6 +63,9 @@ struct nest_prob_state {
 // patch array. Coordinates are extended-space local indices. A broadcast flag
 // repeats the single source row (bi) / column (bj): that is the zero-gradient
 // outflow ghost fill. Periodic and patch-to-patch regions are translations.
+// A mirror flag runs the source box backwards along that dimension and the
+// negation mask flips the sign of the listed variables (bit k: variable k):
+// that is the reflecting-wall fill (mirror image, normal momentum negated).
 struct nest_region {
     long long dst_off, src_off;  // patch array offsets (elements)
     int dst_nye, src_nye;        // extended row lengths (dim 1)
@@ -70,6 +73,8 @@ struct nest_region {
     int di, dj, ni, nj;          // destination box
     int si, sj;                  // source box start
     int bi, bj;                  // broadcast flags
+    int mi, mj;                  // mirror flags
+    int neg;                     // variables negated (bit mask)
 };
 
 // gamma-law EOS constants: gamma, gm1 = gamma - 1.0 (evaluated as the
@@ -86,14 +91,25 @@ struct nest_item {
     int patch, c0, r0, r1;
 };
 
-// Global index mapped through a boundary condition: periodic wrap or
-// zero-gradient (outflow) clamp.
+// Global index mapped through a boundary condition: periodic wrap,
+// zero-gradient (outflow) clamp, or reflecting-wall mirror (ghost cell -1-k
+// is the image of cell k, n+k of n-1-k). Indices past the mirror image of
+// the whole line (only lanes far past the domain reach them, and their
+// values are not used) are clamped into range.
 NEST_HD inline int nest_map_index(int g, int n, int bc) {
     if (bc == 0) {
         int r = g % n;
         return r < 0 ? r + n : r;
     }
+    if (bc == 2) g = g < 0 ? -1 - g : (g >= n ? 2 * n - 1 - g : g);
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+// Variables a gathered value negates relative to its source cell: bit 1
+// (rho u) past a reflecting x wall, bit 2 (rho v) past a reflecting y wall.
+NEST_HD inline int nest_gather_flip(const nest_patch_desc& pd, const nest_prob_geom& G, int e, int g) {
+    const int gr = pd.x0 + e - 2;
+    return (G.bcx == 2 && (gr < 0 || gr >= G.NX) ? 2 : 0) | (G.bcy == 2 && (g < 0 || g >= G.NY) ? 4 : 0);
 }
 
 // Where the value of extended row e of patch p, at global column g of its

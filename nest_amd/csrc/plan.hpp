@@ -1,6 +1,6 @@
 // Host-side construction of the device tables of a 2D problem set: the
 // tensor-product patch partition, the per-problem geometry and owner tables,
-// the halo exchange plan (faces + corners, periodic / outflow /
+// the halo exchange plan (faces + corners, periodic / outflow / reflecting /
 // patch-to-patch) and the stage work lists. Pure C++ (no HIP), so the exact
 // tables the GPU consumes are also checked on the CPU (tests/cpp/
 // test_plan.cpp runs them against the oracle's halo fill).
@@ -12,7 +12,7 @@
 
 namespace nest_plan {
 
-enum { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2 };
 
 // Tensor-product partition of one dimension with subspace() semantics
 // (reference include/nest/core.hpp:184-199).
@@ -31,11 +31,13 @@ inline dim_part partition(int total, int n) {
 }
 
 // source of the ghost strip on `side` (-1, 0, +1) of partition p along one
-// dimension: (source partition, first source extended index, broadcast)
-inline void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& s, int& bcast) {
+// dimension: (source partition, first source extended index, broadcast,
+// mirror). A reflecting wall mirrors the first / last h interior cells.
+inline void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& s, int& bcast, int& mirror) {
     const int np = (int)d.len.size();
     const int h = 2;
     bcast = 0;
+    mirror = 0;
     if (side == 0) {
         q = p;
         s = h;
@@ -46,7 +48,8 @@ inline void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& 
         } else {
             q = p;
             s = h;
-            bcast = 1;
+            bcast = bc == BC_OUTFLOW;
+            mirror = bc == BC_REFLECTING;
         }
     } else {
         if (p < np - 1 || bc == BC_PERIODIC) {
@@ -54,8 +57,9 @@ inline void dim_source(int side, int p, const dim_part& d, int bc, int& q, int& 
             s = h;
         } else {
             q = p;
-            s = h + d.len[p] - 1;
-            bcast = 1;
+            s = bc == BC_REFLECTING ? d.len[p] : h + d.len[p] - 1;
+            bcast = bc == BC_OUTFLOW;
+            mirror = bc == BC_REFLECTING;
         }
     }
 }
@@ -75,8 +79,9 @@ inline void add_mesh_regions(std::vector<nest_region>& R, const dim_part& X, con
                     if (sx == 0 && sy == 0) continue;
                     nest_region g{};
                     int qx, qy;
-                    dim_source(sx, px, X, bcx, qx, g.si, g.bi);
-                    dim_source(sy, py, Y, bcy, qy, g.sj, g.bj);
+                    dim_source(sx, px, X, bcx, qx, g.si, g.bi, g.mi);
+                    dim_source(sy, py, Y, bcy, qy, g.sj, g.bj, g.mj);
+                    g.neg = (g.mi ? 2 : 0) | (g.mj ? 4 : 0);  // normal momenta of mirrored directions
                     const int q = qy * npx + qx;
                     g.di = sx < 0 ? 0 : (sx == 0 ? h : h + lx);
                     g.ni = sx == 0 ? lx : h;
@@ -194,10 +199,11 @@ inline void execute_regions_host(double* base, const std::vector<nest_region>& R
         for (int k = 0; k < nvars; ++k)
             for (int ia = 0; ia < g.ni; ++ia)
                 for (int jb = 0; jb < g.nj; ++jb) {
-                    const int si = g.si + (g.bi ? 0 : ia);
-                    const int sj = g.sj + (g.bj ? 0 : jb);
+                    const int si = g.si + (g.bi ? 0 : (g.mi ? g.ni - 1 - ia : ia));
+                    const int sj = g.sj + (g.bj ? 0 : (g.mj ? g.nj - 1 - jb : jb));
+                    const double v = base[g.src_off + (long long)k * g.src_vs + (long long)si * g.src_nye + sj];
                     base[g.dst_off + (long long)k * g.dst_vs + (long long)(g.di + ia) * g.dst_nye + g.dj + jb] =
-                        base[g.src_off + (long long)k * g.src_vs + (long long)si * g.src_nye + sj];
+                        (g.neg >> k) & 1 ? -v : v;
                 }
 }
 

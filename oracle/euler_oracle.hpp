@@ -39,7 +39,12 @@ template <class T> inline T nmin(T a, T b) { return (val(b) < val(a)) ? b : a; }
 template <class T> inline T nmax(T a, T b) { return (val(a) < val(b)) ? b : a; }
 
 enum limiter_kind : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_NONE = 2 };
-enum bc_kind : int { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
+// Boundary conditions. The reference has periodic ones only (its exchange
+// plans, include/nest/patch.hpp:207-326); outflow and reflecting are defined
+// by this build: outflow ghosts copy the boundary cell (zero gradient),
+// reflecting ghosts are the mirror image of the interior with the normal
+// momentum negated (a solid wall).
+enum bc_kind : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2 };
 
 // ---------------------------------------------------------------------------
 // EOS — examples/sod1d/euler1d.hpp:27-62
@@ -178,7 +183,8 @@ void stage_1d(const T* Uin, T* Uout, int n, int h, double dt, double dx, double 
 }
 
 // ghost fill of a single 1D array: periodic as benchmark_euler1d.cpp:102-110,
-// outflow (zero-gradient copy of the boundary cell) as the build defines it.
+// outflow (zero-gradient copy of the boundary cell) and reflecting (mirror
+// image, momentum negated) as the build defines them.
 template <class T>
 void fill_1d(T* U, int n, int h, int nvars, int bc) {
     const int ne = n + 2 * h;
@@ -188,6 +194,10 @@ void fill_1d(T* U, int n, int h, int nvars, int bc) {
             if (bc == BC_PERIODIC) {
                 u[g] = u[n + g];
                 u[h + n + g] = u[h + g];
+            } else if (bc == BC_REFLECTING) {
+                // ghost h-1-j is the image of interior cell j (extended h+j)
+                u[g] = k == 1 ? -u[2 * h - 1 - g] : u[2 * h - 1 - g];
+                u[h + n + g] = k == 1 ? -u[h + n - 1 - g] : u[h + n - 1 - g];
             } else {
                 u[g] = u[h];
                 u[h + n + g] = u[h + n - 1];
@@ -406,13 +416,18 @@ struct mesh2d {
     int pyof(int p) const { return p / npx; }
     static int mapc(int g, int n, int bc) {
         if (bc == BC_PERIODIC) return ((g % n) + n) % n;
+        if (bc == BC_REFLECTING) return g < 0 ? -1 - g : (g >= n ? 2 * n - 1 - g : g);  // |g| within a halo
         return g < 0 ? 0 : (g >= n ? n - 1 : g);
     }
+    // a coordinate past a reflecting wall (its normal momentum is negated)
+    static bool mirrored(int g, int n, int bc) { return bc == BC_REFLECTING && (g < 0 || g >= n); }
 };
 
 // Halo fill over the whole mesh: every ghost cell (faces AND corners) takes the
 // value of the global cell obtained by mapping each coordinate through its
-// boundary condition (periodic wrap / outflow clamp). On faces of a periodic
+// boundary condition (periodic wrap / outflow clamp / reflecting mirror, the
+// last negating the normal momentum of each mirrored direction: rho u for x,
+// rho v for y). On faces of a periodic
 // mesh this is exactly include/nest/patch.hpp:153-190 driven by the plans of
 // build_periodic_mesh_2d (patch.hpp:207-326); corners and outflow are the
 // build-defined extension.
@@ -429,8 +444,10 @@ void fill_mesh(T* U, const mesh2d& m) {
             for (int ej = 0; ej < nye; ++ej) {
                 bool inter = ei >= h && ei < nxe - h && ej >= h && ej < nye - h;
                 if (inter) continue;
-                int gi = m.mapc(m.x0[px] + ei - h, m.nx, m.bcx);
-                int gj = m.mapc(m.y0[py] + ej - h, m.ny, m.bcy);
+                const int ri = m.x0[px] + ei - h, rj = m.y0[py] + ej - h;
+                int gi = m.mapc(ri, m.nx, m.bcx);
+                int gj = m.mapc(rj, m.ny, m.bcy);
+                const bool fx = m.mirrored(ri, m.nx, m.bcx), fy = m.mirrored(rj, m.ny, m.bcy);
                 int qx = m.owner_x[gi], qy = m.owner_y[gj];
                 int q = qy * m.npx + qx;
                 const int qnye = m.ly[qy] + 2 * h;
@@ -438,7 +455,10 @@ void fill_mesh(T* U, const mesh2d& m) {
                 const T* src = U + m.off[q];
                 size_t so = (size_t)(gi - m.x0[qx] + h) * qnye + (gj - m.y0[qy] + h);
                 size_t d_o = (size_t)ei * nye + ej;
-                for (int k = 0; k < 4; ++k) dst[k * vs + d_o] = src[k * qvs + so];
+                for (int k = 0; k < 4; ++k) {
+                    const T v = src[k * qvs + so];
+                    dst[k * vs + d_o] = (k == 1 && fx) || (k == 2 && fy) ? -v : v;
+                }
             }
     }
 }

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libnest_ref.so")
 
 LIM = {"minmod": 0, "mc": 1, "none": 2}
-BC = {"periodic": 0, "outflow": 1}
+BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _d, _i, _sz = C.c_double, C.c_int, C.c_size_t

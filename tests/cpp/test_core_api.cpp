@@ -135,16 +135,18 @@ TEST(periodic_2d_faces_match_wrap_formula) {
 }
 
 TEST(mesh_2d_with_corners_matches_oracle_fill) {
-    // faces + corners, periodic / outflow per dimension, uneven partitions
-    for (int bx = 0; bx < 2; ++bx)
-        for (int by = 0; by < 2; ++by) {
+    // faces + corners, periodic / outflow / reflecting per dimension, uneven
+    // partitions (the oracle's bc codes: 0 periodic, 1 outflow, 2 reflecting)
+    const boundary_t kinds[3] = {boundary_t::periodic, boundary_t::outflow, boundary_t::reflecting};
+    for (int bx = 0; bx < 3; ++bx)
+        for (int by = 0; by < 3; ++by) {
             const int nx = 13, ny = 9, npx = 3, npy = 2;
-            auto bcx = bx ? boundary_t::outflow : boundary_t::periodic;
-            auto bcy = by ? boundary_t::outflow : boundary_t::periodic;
+            auto bcx = kinds[bx];
+            auto bcy = kinds[by];
             auto patches = build_mesh_2d<double>(index_space(ivec(0, 0), uvec(nx, ny)), npx, npy, 4, 2, bcx, bcy);
             nest_oracle::mesh2d m(nx, ny, npx, npy, 2, bx, by);
             std::vector<double> flat(m.total);
-            std::mt19937_64 g(bx * 2 + by);
+            std::mt19937_64 g(bx * 3 + by);
             std::uniform_real_distribution<double> U(-1, 1);
             for (auto& p : patches)
                 for (std::size_t v = 0; v < 4; ++v)

@@ -3,10 +3,11 @@
 //  * the halo-region table, the geometry / owner tables and both work lists
 //    stay inside the buffers;
 //  * executing the region table reproduces the oracle's halo fill (faces +
-//    corners, periodic / outflow) bit for bit;
+//    corners, periodic / outflow / reflecting) bit for bit;
 //  * nest_gather_source (the addressing of the stage kernels' in-kernel halo
 //    gather) names, for every extended cell of every patch, an interior cell
-//    that holds the oracle's ghost value;
+//    that holds the oracle's ghost value (negated where nest_gather_flip
+//    says so);
 //  * the per-patch strips and the global strips each write every interior
 //    cell exactly once.
 // Exit status 0 = pass.
@@ -99,8 +100,10 @@ static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int r
                               lc < qd.ny + 2;
                 CHECK(inside, "gather source not interior p%d e%d ec%d", p, e, ec);
                 if (!inside) continue;
+                const int fl = nest_gather_flip(d, G, e, d.y0 + ec - 2);
                 for (int k = 0; k < 4; ++k) {
                     double got = b[qd.off + (long long)k * (qd.nx + 4) * (qd.ny + 4) + (long long)le * (qd.ny + 4) + lc];
+                    if ((fl >> k) & 1) got = -got;
                     double want = b[d.off + (long long)k * (d.nx + 4) * (d.ny + 4) + (long long)e * (d.ny + 4) + ec];
                     CHECK(got == want, "gather value p%d e%d ec%d", p, e, ec);
                 }
@@ -116,8 +119,8 @@ static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int r
 }
 
 int main() {
-    for (int bcx = 0; bcx < 2; ++bcx)
-        for (int bcy = 0; bcy < 2; ++bcy) {
+    for (int bcx = 0; bcx < 3; ++bcx)  // periodic, outflow, reflecting
+        for (int bcy = 0; bcy < 3; ++bcy) {
             check_mesh(96, 80, 3, 2, bcx, bcy, 64);
             check_mesh(70, 130, 2, 3, bcx, bcy, 17);
             check_mesh(40, 200, 1, 1, bcx, bcy, 64);

@@ -244,3 +244,15 @@ def test_pack_unpack_roundtrip():
     sim.synchronize()
     assert np.array_equal(sim.gather(host(back)), G)
     sim.close()
+
+
+def test_adjoint_rejects_reflecting_walls():
+    """Reflecting walls are forward-only: the adjoint entry points return
+    NEST_ERR_UNSUPPORTED and leave the output untouched."""
+    from nest_amd._lib import NestError
+    print_case("adjoint reflecting rejected")
+    sim = Sim2D.mesh(32, 24, 2, 1, bcx="reflecting", bcy="periodic", mode="fast")
+    U = sim.scatter(rough_state(32, 24, seed=5))
+    with pytest.raises(NestError) as e:
+        gpu_stage_vjp(sim, U, U, 1e-4)
+    assert e.value.code == 5 and "reflecting" in str(e.value)

@@ -19,7 +19,7 @@ TOL_FAST = 1e-12
 # 1D
 # ----------------------------------------------------------------------------
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-@pytest.mark.parametrize("bc", ["outflow", "periodic"])
+@pytest.mark.parametrize("bc", ["outflow", "periodic", "reflecting"])
 def test_1d_sod_lockstep(oracle, mode, bc):
     """1D Sod, N=1000, CFL 0.4: same U^n and dt fed to both each step."""
     n, cfl = 1000, 0.4
@@ -130,6 +130,10 @@ CASES_2D = [
     (64, 64, 2, 2, "periodic", "periodic", "quad"),
     (70, 130, 2, 3, "periodic", "outflow", "quad"),
     (40, 200, 1, 1, "outflow", "periodic", "sod"),
+    # reflecting walls (normal momentum negated in the mirrored ghosts)
+    (72, 56, 3, 2, "reflecting", "outflow", "quad"),
+    (64, 96, 2, 3, "periodic", "reflecting", "quad"),
+    (50, 90, 1, 1, "reflecting", "reflecting", "quad"),
 ]
 
 
@@ -179,7 +183,7 @@ def test_2d_mesh_lockstep(oracle, case, mode, kernel):
 
 
 def test_2d_halo_fill_bitwise(oracle):
-    """Device exchange plan (faces + corners, periodic and outflow) == oracle."""
+    """Device exchange plan (faces + corners; periodic, outflow, reflecting) == oracle."""
     for (nx, ny, npx, npy, bcx, bcy, _) in CASES_2D:
         print_case("halo", nx, ny, npx, npy, bcx, bcy)
         sim = Sim2D.mesh(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
@@ -262,6 +266,42 @@ def test_batch_matches_single_problems(oracle):
         kk, dts = oracle.mesh_run(u, m, 5, 1.0 / m.nx, 1.0 / m.ny, cfl=0.4)
         assert st["steps"][k] == 5
         assert np.array_equal(sim.patch_view(got, k)[:, 2:-2, 2:-2], u.reshape(4, m.nx + 4, m.ny + 4)[:, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_batch_reflecting_walls(oracle, mode):
+    """Ensemble in closed boxes (all walls reflecting): single-patch problems
+    read their ghosts straight from the interior, mirrored with the normal
+    momentum negated. Device-dt run == the oracle's loop per problem; mass and
+    energy of each box stay constant."""
+    sizes = [(48, 64), (81, 81), (100, 52)]
+    print_case("batch reflecting", sizes, mode)
+    sim = Sim2D.batch(sizes, bc="reflecting", mode=mode)
+    U = np.zeros(sim.total)
+    refs = []
+    for k, (nx, ny) in enumerate(sizes):
+        m = MeshSpec(nx, ny, 1, 1, bcx="reflecting", bcy="reflecting")
+        G = P.quadrants(max(nx, ny), k + 11)[:, :nx, :ny].copy()
+        u = m.to_patches(G)
+        oracle.mesh_fill(u, m)
+        sim.patch_view(U, k)[:] = u.reshape(4, nx + 4, ny + 4)
+        refs.append((m, u, G))
+    sim.upload(U)
+    sim.start(cfl=0.4)
+    sim.advance(8, graph=False)
+    got = sim.download()
+    st = sim.status()
+    for k, (m, u, G) in enumerate(refs):
+        oracle.mesh_run(u, m, 8, 1.0 / m.nx, 1.0 / m.ny, cfl=0.4)
+        assert st["steps"][k] == 8
+        g = sim.patch_view(got, k)[:, 2:-2, 2:-2]
+        w = u.reshape(4, m.nx + 4, m.ny + 4)[:, 2:-2, 2:-2]
+        if mode == "exact":
+            assert np.array_equal(g, w), k
+        else:
+            assert rel_linf_vars(g, w) <= 1e-11, k  # 8 free-running steps
+        for var in (0, 3):
+            assert abs(g[var].sum() - G[var].sum()) <= 1e-12 * G[var].sum()
 
 
 def test_per_patch_path_equals_batched(oracle):

@@ -41,6 +41,15 @@ def test_error_reporting_without_device():
     rc = L.nest_hip_mesh2d_create(8, 8, 8, 1, 0.1, 0.1, 1.4, 0, 0, 0, 0, C.byref(h))
     assert rc == 1 and "2 interior cells" in _lib.last_error()
     assert L.nest_hip_sim2d_info(None, None, None, None, None) == 1
+    rc = L.nest_hip_mesh2d_create(8, 8, 1, 1, 0.1, 0.1, 1.4, 3, 0, 0, 0, C.byref(h))
+    assert rc == 1 and "2 (reflecting)" in _lib.last_error()
+
+
+def test_adjoint_rollout_rejects_reflecting_before_device_work():
+    from nest_amd.adjoint import Rollout
+    for make in (lambda: Rollout.mesh(16, 16, bc="reflecting"), lambda: Rollout.batch(2, 16, 16, bc="reflecting")):
+        with pytest.raises(ValueError, match="reflecting"):
+            make()
 
 
 def test_no_cpu_fallback_when_library_missing(tmp_path):
