@@ -4,9 +4,11 @@
 independent problems) on the GPU through the C ABI and is differentiable
 with respect to the initial conserved state: the backward pass is the
 library's hand-written discrete adjoint (nest_hip_sim2d_rollout_adjoint),
-which reverses the stored rollout interval by interval, recomputing each
-interval's states from its checkpoint (checkpoints every ceil(sqrt(K))
-steps by default). Derivatives are exact for the discrete map, branch by
+which reverses the stored rollout. By default every state of the rollout
+is kept in device memory when 2K states fit in half of the free memory (the
+reverse then recomputes nothing); otherwise it goes interval by interval,
+recomputing each interval's states from its checkpoint (checkpoints every
+ceil(sqrt(K)) steps). Derivatives are exact for the discrete map, branch by
 branch (limiter cases, HLLE upwind / subsonic branch, Davis speeds, floors,
 the periodic / zero-gradient halo fill).
 
@@ -38,6 +40,7 @@ class Rollout:
         self.sim = sim
         self.shape = tuple(shape)
         self.nsteps, self.dt, self.interval = int(nsteps), float(dt), int(interval)
+        self._auto = None
         self._token = 0
         self.stream = torch.cuda.Stream()  # default priority
         sim.set_stream(self.stream.cuda_stream)
@@ -67,6 +70,21 @@ class Rollout:
             ro.tune([(nx, ny)] * nprob)
         return ro
 
+    def rollout_interval(self) -> int:
+        """The interval handed to the library: `interval` as given (> 0:
+        checkpoints every `interval` steps; < 0: every state kept), except 0,
+        the automatic choice: every state kept (-1) when the 2K states it
+        needs fit in half of the free device memory, else checkpoints every
+        ceil(sqrt(K)) steps (0). The choice is made at the first call and kept,
+        so later calls reuse the buffers it allocated."""
+        if self.interval != 0:
+            return self.interval
+        if self._auto is None:
+            need = 2 * self.nsteps * self.sim.total * 8
+            free, _ = torch.cuda.mem_get_info()
+            self._auto = -1 if need <= free // 2 else 0
+        return self._auto
+
     def tune(self, sizes, log=None):
         """Autotune the forward stage and adjoint kernels' tiling on this
         handle (results cached per device and problem-size signature). The
@@ -76,7 +94,7 @@ class Rollout:
         from . import autotune
         self.sim.start(cfl=0.4)
         fwd = autotune.tune(self.sim, sizes, log=log)
-        adj = autotune.tune_adjoint(self.sim, sizes, self.dt, log=log)
+        adj = autotune.tune_adjoint(self.sim, sizes, self.dt, log=log, keep_all=self.rollout_interval() < 0)
         return fwd, adj
 
     def __call__(self, U0: torch.Tensor) -> torch.Tensor:
@@ -98,7 +116,7 @@ class _RolloutFn(torch.autograd.Function):
         ro.stream.wait_stream(cur)
         Uptr, _ = sim.device_ptrs()
         sim.unpack(U0c.data_ptr(), Uptr)
-        sim.rollout(ro.nsteps, ro.dt, ro.interval)
+        sim.rollout(ro.nsteps, ro.dt, ro.rollout_interval())
         sim.pack(Uptr, out.data_ptr())
         for t in (U0c, out):
             t.record_stream(ro.stream)

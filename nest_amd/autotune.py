@@ -88,16 +88,19 @@ def _store(path, table):
         pass
 
 
-def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 5, cache: str | None = None, log=None):
+def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 5, cache: str | None = None, log=None,
+                 keep_all: bool = False):
     """Tune the adjoint kernels' tiling of `sim` in place: coordinate descent
     over rows and waves per block of the Y and X kernels, each candidate
-    timed as the fastest of `reps` runs of an 8-step rollout (checkpoints
-    every 4 steps) and its reverse sweep, from the handle's own state
-    (advanced by each run) with its stage scratch as the cotangent, so no
-    extra memory. Returns {"rows_y", "waves_y", "rows_x", "waves_x",
-    "ms_per_step"}."""
+    timed as the fastest of `reps` runs of an 8-step rollout and its reverse
+    sweep, from the handle's own state (advanced by each run) with its stage
+    scratch as the cotangent. The rollout is stored the way the caller's
+    rollouts will be: every state kept (`keep_all`; the reverse runs alone),
+    or checkpoints every 4 steps (the recompute of the first interval shares
+    the device with the reverse of the second). Returns {"rows_y", "waves_y",
+    "rows_x", "waves_x", "ms_per_step"}."""
     classes = size_classes(sizes)
-    key = "adjoint-rollout|" + signature(sizes, classes, arch)
+    key = ("adjoint-rollout-kept|" if keep_all else "adjoint-rollout|") + signature(sizes, classes, arch)
     path = _cache_path(cache)
     table = _load(path)
     if key in table:
@@ -108,7 +111,7 @@ def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 5, cac
     nsteps = 8
 
     def once():
-        sim.rollout(nsteps, dt, 4)
+        sim.rollout(nsteps, dt, -1 if keep_all else 4)
         sim.rollout_adjoint(U1)
 
     def run(c):

@@ -954,13 +954,17 @@ int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double
 int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
     if (int rc = valid_sim(s)) return rc;
     if (nsteps < 1 || !(dt > 0.0)) return fail(NEST_ERR_INVALID, "nsteps >= 1 and dt > 0");
-    const int m = interval > 0 ? interval : std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
+    // interval < 0 keeps every state: one interval of nsteps, kept whole
+    const bool all = interval < 0;
+    const int m = all ? nsteps : (interval > 0 ? interval : std::max(1, (int)std::ceil(std::sqrt((double)nsteps))));
     const int nck = (nsteps + m - 1) / m;
     if (int rc = s->grow(s->ckpt, (size_t)nck)) return rc;
-    // With two or more intervals, the states of the last interval (and their
-    // stage-1 states) are written straight into the buffer set its reverse
-    // reads, so the reverse sweep starts without recomputing it.
-    const bool keep = nck >= 2;
+    // With two or more intervals (or every state kept), the states of the
+    // last interval (and their stage-1 states) are written straight into the
+    // buffer set its reverse reads, so the reverse sweep starts without
+    // recomputing it. A single interval of a plain forward (interval >=
+    // nsteps) keeps nothing: its reverse recomputes it from the initial state.
+    const bool keep = all || nck >= 2;
     const int nsets = std::min(NEST_ADJ_SETS, nck);
     const int cl = nck - 1, setl = cl % nsets, lenl = nsteps - cl * m;
     if (keep) {
@@ -1034,19 +1038,23 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     // everything enqueued before (the rollout and its checkpoints) comes first
     NEST_HIP_TRY(hipEventRecord(s->ev_start, s->stream));
     NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_start, 0));
-    // the last interval's states were kept by the rollout (with 2+ intervals)
-    const bool kept = s->roll_kept && nck >= 2;
+    // the last interval's states were kept by the rollout (with 2+ intervals,
+    // or every state)
+    const bool kept = s->roll_kept;
     for (int c = nck - 1; c >= std::max(0, nck - (nsets - 1)); --c)
         if (!(kept && c == nck - 1)) NEST_HIP_TRY(recompute(c));
-    // reverse of interval c (handle's stream), then the recompute S - 1
-    // intervals further down (auxiliary stream; it waits for the reverse
-    // that last read its buffer set)
+    // the recompute S - 1 intervals further down (auxiliary stream; it waits
+    // for the reverse of interval c + 1, the last to read its buffer set),
+    // then the reverse of interval c (handle's stream). The recompute goes
+    // into the queue first, so the auxiliary stream has it in hand when the
+    // reverse of c starts, however far ahead of the device the host is.
     for (int c = nck - 1; c >= 0; --c) {
         const int set = c % nsets, len = std::min(m, K - c * m);
+        const int r = c - (nsets - 1);
+        if (r >= 0 && !(kept && r == nck - 1)) NEST_HIP_TRY(recompute(r));
         if (!(kept && c == nck - 1)) NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
         for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
         NEST_HIP_TRY(hipEventRecord(s->ev_rev_c[c], s->stream));
-        if (c - (nsets - 1) >= 0) NEST_HIP_TRY(recompute(c - (nsets - 1)));
     }
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;

@@ -291,7 +291,8 @@ class Sim2D:
 
     def rollout(self, nsteps: int, dt: float, interval: int = 0) -> None:
         """nsteps fixed-dt steps of the state with checkpoints every
-        `interval` steps (0: ceil(sqrt(nsteps)))."""
+        `interval` steps (0: ceil(sqrt(nsteps)); < 0: every state kept, so
+        the reverse recomputes nothing; >= nsteps: a plain forward)."""
         check(self.L.nest_hip_sim2d_rollout(self.h, nsteps, dt, interval))
 
     def rollout_adjoint(self, Ubar: int) -> None:

@@ -191,12 +191,14 @@ def test_stage_and_step_dot_product(oracle, case, mode):
     sim.close()
 
 
-@pytest.mark.parametrize("interval", [0, 1, 4])
+@pytest.mark.parametrize("interval", [0, 1, 4, 16, -1])
 def test_rollout_checkpointed_adjoint(oracle, interval):
     """K = 11 fixed-dt steps with checkpoints every `interval` steps (0:
-    ceil(sqrt K) = 4): the final state is bitwise the oracle's 11 steps, and
-    the reverse sweep (recompute per interval) satisfies the dot-product
-    identity against the dual-number JVP of the 11-step map."""
+    ceil(sqrt K) = 4; 16: one interval, recomputed whole before its reverse;
+    -1: every state kept, nothing recomputed):
+    the final state is bitwise the oracle's 11 steps, and the reverse sweep
+    (recompute per interval) satisfies the dot-product identity against the
+    dual-number JVP of the 11-step map."""
     nx, ny, npx, npy, K = 64, 48, 2, 2, 11
     print_case("adjoint rollout", interval)
     m = MeshSpec(nx, ny, npx, npy, bcx="periodic", bcy="periodic")
@@ -207,7 +209,7 @@ def test_rollout_checkpointed_adjoint(oracle, interval):
     sim.upload(U)
     sim.rollout(K, dt, interval)
     k, mm, _ = sim.rollout_info()
-    assert k == K and mm == (interval or 4)
+    assert k == K and mm == {0: 4, -1: K}.get(interval, interval)
     final = sim.gather(sim.download())
     rng = np.random.default_rng(22)
     v = rng.standard_normal((4, nx, ny))

@@ -99,6 +99,24 @@ def _dual_check(oracle, U0, grad, target, n, npx, npy, K, dt, ndir, seed):
     return worst
 
 
+def test_gradient_independent_of_state_storage():
+    """The same gradient, bitwise, whether the rollout keeps every state
+    (interval -1, and the automatic choice at this size), checkpoints every 5
+    steps (recompute per interval, last interval kept), or keeps only the
+    initial state (interval >= K: the whole rollout recomputed); and the
+    automatic choice keeps every state when it fits."""
+    print_case("gradient vs state storage 96^2 K=12")
+    grads = {}
+    for interval in (-1, 0, 5, 12):
+        ro, U0, U0d, loss, grad, target, dt = _setup(96, 12, "fast", npx=2, npy=3, interval=interval)
+        grads[interval] = grad.detach().cpu().numpy()
+        if interval == 0:
+            assert ro.rollout_interval() == -1
+        ro.close()
+    for interval in (0, 5, 12):
+        assert np.array_equal(grads[interval], grads[-1]), interval
+
+
 def test_torch_gradient_small(oracle):
     """96^2, 2x3 patches, K = 12, checkpoints every 5 steps: dual numbers at
     the smooth initial state, central differences at the generic point."""
@@ -115,11 +133,13 @@ def test_torch_gradient_small(oracle):
 
 def test_adjoint1024_gradient_vs_central_fd():
     """Full size: 1024^2 periodic, K = 200 fixed-dt steps,
-    sqrt(K) = 15-step checkpoint intervals, 16 seeded smooth directions of
-    the full state at the generic point (fast numerics). The agreement at
-    the smooth initial state itself along density-only directions is printed."""
+    sqrt(K) = 15-step checkpoint intervals (the recomputing reverse; the
+    default keeps every state at this size, and gives the same gradient bit
+    for bit), 16 seeded smooth directions of the full state at the generic
+    point (fast numerics). The agreement at the smooth initial state itself
+    along density-only directions is printed."""
     print_case("adjoint gradient vs FD, 1024^2 K=200")
-    ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "fast", npx=4, npy=4,
+    ro, U0, U0d, loss, grad, target, dt = _setup(1024, 200, "fast", npx=4, npy=4, interval=15,
                                                  U0=P.adjoint_generic_state(1024))
     k, m, _ = ro.sim.rollout_info()
     assert k == 200 and m == 15
