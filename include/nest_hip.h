@@ -194,8 +194,8 @@ int nest_hip_sim2d_synchronize(nest_sim2d* s);
 /* ---- discrete adjoint of fixed-dt rollouts ---------------------------------
    No counterpart in the reference (it has no differentiation). These are the
    rollout_fwd / rollout_adj entry points behind the torch.autograd.Function
-   (nest_amd/adjoint.py). The differentiated map is the build's RK2 step at a
-   fixed dt, U1 = S(U), U' = 0.5 (U + S(U1)), S = Y o fill o X o fill of
+   (nest_amd/adjoint.py). The differentiated map is the library's RK2 step at
+   a fixed dt, U1 = S(U), U' = 0.5 (U + S(U1)), S = Y o fill o X o fill of
    x_sweep_t / y_sweep_t (examples/sod2d/euler2d.hpp:67-239) with the halo
    fill (periodic wrap / zero-gradient copy) included. Derivatives are those
    of the branch the primal takes (limiter cases, Davis speeds, HLLE upwind or

@@ -18,7 +18,10 @@
 // against vectors produced by the reference's own headers compiled in
 // oracle/_ref (committed under tests/golden/), against the reference's
 // known-answer tests (tests/test_euler.cpp, tests/test_core.cpp), and against
-// invariants (free-stream preservation, conservation, mirror symmetry).
+// invariants (free-stream preservation, conservation, mirror symmetry). The
+// reflecting wall, which the reference lacks, is pinned by the method of
+// images (a walled box evolves as the mirror-doubled periodic box), closed-box
+// conservation and brute-force image ghosts (tests/test_oracle_reflecting.py).
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -40,8 +43,8 @@ template <class T> inline T nmax(T a, T b) { return (val(a) < val(b)) ? b : a; }
 
 enum limiter_kind : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_NONE = 2 };
 // Boundary conditions. The reference has periodic ones only (its exchange
-// plans, include/nest/patch.hpp:207-326); outflow and reflecting are defined
-// by this build: outflow ghosts copy the boundary cell (zero gradient),
+// plans, include/nest/patch.hpp:207-326); outflow and reflecting are
+// extensions: outflow ghosts copy the boundary cell (zero gradient),
 // reflecting ghosts are the mirror image of the interior with the normal
 // momentum negated (a solid wall).
 enum bc_kind : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2 };
@@ -429,8 +432,8 @@ struct mesh2d {
 // last negating the normal momentum of each mirrored direction: rho u for x,
 // rho v for y). On faces of a periodic
 // mesh this is exactly include/nest/patch.hpp:153-190 driven by the plans of
-// build_periodic_mesh_2d (patch.hpp:207-326); corners and outflow are the
-// build-defined extension.
+// build_periodic_mesh_2d (patch.hpp:207-326); corners, outflow and reflecting
+// walls extend it.
 template <class T>
 void fill_mesh(T* U, const mesh2d& m) {
     const int h = m.h;
@@ -509,7 +512,8 @@ void split_stage_mesh(T* U, T* scratch, const mesh2d& m, double dt, double dx, d
     copy_interior(scratch, U, m);
 }
 
-// Build-defined 2D SSP-RK2: U1 = S(Un); U2 = S(U1); U^{n+1} = 0.5 (Un + U2)
+// 2D SSP-RK2 (no 2D RK2 driver exists on the reference's CPU path):
+// U1 = S(Un); U2 = S(U1); U^{n+1} = 0.5 (Un + U2)
 // (the "2 stages x 2 sweeps" step of docs/PERFORMANCE.md:25-28 and
 // src/backend_cuda/euler2d_cuda.cuh:610-630). The result's halos are refilled
 // so the returned state is halo-complete.

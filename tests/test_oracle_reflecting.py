@@ -1,7 +1,7 @@
 """Reflecting walls in the oracle, pinned by what the mathematics fixes.
 
 The reference has periodic and outflow boundaries only; the reflecting wall is
-this build's extension (ghost cells mirror the interior across the wall, the
+an extension (ghost cells mirror the interior across the wall, the
 wall-normal momentum negated). Nothing in the reference's tests covers it, so
 the oracle's wall is pinned by:
 
@@ -9,8 +9,8 @@ the oracle's wall is pinned by:
   periodic problem on the mirror-doubled box whose second half is the mirror
   image of the first (up to the round-off by which the scheme is not bitwise
   mirror-symmetric);
-- a closed box conserves mass, energy and the wall-tangential momentum: the
-  wall flux of each vanishes for mirror-image face states;
+- a closed box conserves mass and energy: the wall flux of each vanishes for
+  mirror-image face states;
 - ghost values against a brute-force image construction.
 
 A ghost taken from the wrong interior cell, a missed or extra negation, or a
