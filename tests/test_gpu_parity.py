@@ -182,6 +182,69 @@ def test_2d_mesh_lockstep(oracle, case, mode, kernel):
     assert worst <= TOL_FAST
 
 
+BCS = ["periodic", "outflow", "reflecting"]
+
+
+@pytest.mark.parametrize("kernel", STAGE_KERNELS)
+def test_2d_minimum_size_patches(oracle, kernel):
+    """Patches of 2-3 cells (the minimum is 2: the halo width) and tiny
+    single-patch problems, every pair of boundary conditions, exact numerics:
+    3 lockstep steps bitwise equal to the oracle. A 64-lane strip then spans
+    many patches and runs past the problem's edge on both sides, so every
+    lane's source comes through the wrap / clamp / mirror of its column."""
+    shapes = [(9, 7, 4, 3), (2, 3, 1, 1), (5, 2, 2, 1), (3, 70, 1, 4)]
+    for (nx, ny, npx, npy) in shapes:
+        for bcx in BCS:
+            for bcy in BCS:
+                print_case("2d minimum patches", (nx, ny, npx, npy), bcx, bcy, kernel)
+                dx, dy = 1.0 / nx, 1.0 / ny
+                sim = Sim2D.mesh(nx, ny, npx, npy, dx, dy, bcx=bcx, bcy=bcy, mode="exact")
+                sim.set_stage_kernel(kernel)
+                m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+                G = P.quadrants(max(nx, ny), nx * 7 + ny)[:, :nx, :ny].copy()
+                U = sim.scatter(G)
+                oracle.mesh_fill(U, m)
+                for k in range(3):
+                    sim.upload(U)
+                    dt = oracle.mesh_dt(U, m, dx, dy, 0.4)
+                    assert sim.compute_dt(0.4)[0] == dt
+                    sim.rk2_step(dt)
+                    ref = U.copy()
+                    oracle.mesh_rk2_step(ref, m, dt, dx, dy)
+                    got = sim.download()
+                    assert np.array_equal(got, ref), f"step {k}: max diff {np.abs(got - ref).max()}"
+                    U = ref
+                sim.close()
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_batch_of_tiny_problems(oracle, bc):
+    """A batch mixing 2-cell-wide problems with larger ones (device dt, 4
+    steps, exact numerics) == each problem alone in the oracle."""
+    sizes = [(2, 2), (3, 5), (2, 9), (17, 2), (40, 33)]
+    print_case("batch tiny", bc)
+    sim = Sim2D.batch(sizes, bc=bc, mode="exact")
+    U = np.zeros(sim.total)
+    refs = []
+    for k, (nx, ny) in enumerate(sizes):
+        m = MeshSpec(nx, ny, 1, 1, bcx=bc, bcy=bc)
+        u = m.to_patches(P.quadrants(max(nx, ny), k + 3)[:, :nx, :ny].copy())
+        oracle.mesh_fill(u, m)
+        sim.patch_view(U, k)[:] = u.reshape(4, nx + 4, ny + 4)
+        refs.append((m, u))
+    sim.upload(U)
+    sim.start(cfl=0.4)
+    sim.advance(4, graph=False)
+    got = sim.download()
+    st = sim.status()
+    for k, (m, u) in enumerate(refs):
+        oracle.mesh_run(u, m, 4, 1.0 / m.nx, 1.0 / m.ny, cfl=0.4)
+        assert st["steps"][k] == 4
+        g = sim.patch_view(got, k)[:, 2:-2, 2:-2]
+        assert np.array_equal(g, u.reshape(4, m.nx + 4, m.ny + 4)[:, 2:-2, 2:-2]), k
+    sim.close()
+
+
 def test_2d_halo_fill_bitwise(oracle):
     """Device exchange plan (faces + corners; periodic, outflow, reflecting) == oracle."""
     for (nx, ny, npx, npy, bcx, bcy, _) in CASES_2D:
