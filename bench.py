@@ -267,6 +267,51 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
             "fd_max_rel_err_per_direction": worst}
 
 
+def oned_leg(large_steps: int = 200):
+    """1D Sod with device dt through Solver1D.run (two launches per step,
+    replayed 64 steps at a time from a captured graph):
+    * the configuration of the reference's published 1D GPU figure
+      (examples/benchmark/benchmark_euler1d.cpp: N = 1e5, periodic, CFL 0.4,
+      t = 0.2; exact numerics, as the euler1d_gpu_* drop-in), median of 3 runs
+      after one untimed run;
+    * N = 2^24 cells (outflow, fast numerics), about `large_steps` steps: the
+      stage kernels' bandwidth, 120 B per cell-step (stage 1 reads and writes
+      3 doubles, stage 2 reads 6 and writes 3)."""
+    from nest_amd import Solver1D, problems as P
+    out = {}
+    n = 100000
+    U = P.sod1d(n)
+    s = Solver1D(n, 1.0 / n, bc="periodic", mode="exact")
+    times, steps = [], 0
+    for r in range(4):
+        s.upload(U)
+        t0 = time.perf_counter()
+        steps = s.run(0.2, 0.4)
+        times.append(time.perf_counter() - t0)
+    s.close()
+    t = sorted(times[1:])[1]
+    out["sod_1e5"] = {"cells": n, "t_end": 0.2, "bc": "periodic", "numerics": "exact", "steps": steps,
+                      "ms": round(t * 1e3, 2), "us_per_step": round(t * 1e6 / steps, 2),
+                      "mcups": round(n * 2.0 * steps / t / 1e6, 1),
+                      "reference_published_mcups": 5000.0}
+    n = 1 << 24
+    U = P.sod1d(n)
+    s = Solver1D(n, 1.0 / n, bc="outflow", mode="fast")
+    s.upload(U)
+    t_end = large_steps * s.compute_dt(0.4)  # about large_steps steps (the state stays near the first dt)
+    s.run(t_end, 0.4)  # untimed: captures the step graph
+    s.upload(U)
+    t0 = time.perf_counter()
+    k = s.run(t_end, 0.4)
+    t = time.perf_counter() - t0
+    s.close()
+    gbs = 120.0 * n * k / t / 1e9
+    out["large"] = {"cells": n, "bc": "outflow", "numerics": "fast", "steps": k, "ms": round(t * 1e3, 2),
+                    "mcups": round(n * 2.0 * k / t / 1e6, 1), "algorithmic_gbs": round(gbs, 1),
+                    "hbm_frac": round(gbs / 8000.0, 4)}
+    return out
+
+
 def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
     """Kelvin-Helmholtz 8192^2 periodic on 64x64 patches of 128^2,
     `steps` RK2 steps with device dt (graph-replayed); then `nadj`
@@ -374,6 +419,7 @@ def main():
     ap.add_argument("--adjoint-fd-dirs", type=int, default=16,
                     help="central-difference directions of the adjoint gradient check; 0 skips the check")
     ap.add_argument("--kh-steps", type=int, default=100, help="KH 8192^2 + 64 batched 256^2 adjoints leg; 0 skips it")
+    ap.add_argument("--oned-steps", type=int, default=200, help="1D leg (N=1e5 to t=0.2; N=2^24 this many steps); 0 skips it")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling aid: one plain device copy of this many MiB after the legs")
     args = ap.parse_args()
@@ -483,6 +529,11 @@ def main():
         log("bench: KH 8192^2, 64 batched 256^2 adjoints")
         c5 = kh_leg(args.mode, steps=args.kh_steps)
 
+    oned = None
+    if args.oned_steps > 0:
+        log("bench: 1D Sod (N=1e5 to t=0.2; N=2^24)")
+        oned = oned_leg(args.oned_steps)
+
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -537,6 +588,7 @@ def main():
         "ensemble": ens,
         "adjoint": adj,
         "kh_batched_adjoints": c5,
+        "oned": oned,
         "roofline": {"bound": "hbm", "kernel": "stage2d_dma_kernel<minmod, 2>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),

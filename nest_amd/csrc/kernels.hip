@@ -712,25 +712,11 @@ __global__ __launch_bounds__(256) void fill_regions_kernel(double* base, const n
 //    is monotone), init 1e10, cells with s == 0 skipped;
 // 1D compute_dt (sod1d.cpp:127): (cfl*dx)/max(|u|+c);
 // then dt = min(dt, t_end - t) (sod1d.cpp:222, sod2d.cpp:320).
-__global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
-    const int i = blockIdx.x;  // one wave per problem
-    const int lane = threadIdx.x;
-    if (i >= nprob) return;
-    unsigned long long vx = 0, vy = 0;
-    if (slots) {  // fold and clear the stage-2 slots (lane k: slot k)
-        unsigned long long* sl = slots + ((size_t)i * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
-        vx = sl[0];
-        vy = sl[1];
-        sl[0] = 0ull;
-        sl[1] = 0ull;
-    }
-    vx = wave_max_u64(vx);
-    vy = wave_max_u64(vy);
-    if (lane != 0) return;
-    nest_prob_state& s = P[i];
-    // speeds2d / the 1D kernels reduce straight into smax
-    const unsigned long long bx = s.smax[0] > vx ? s.smax[0] : vx;
-    const unsigned long long by = s.smax[1] > vy ? s.smax[1] : vy;
+// Schedule the next step of one problem from the max wave speeds (bit
+// patterns of |u|+c, |v|+c) of its newest state: clear them, then set dt
+// (CFL, fixed, clamped to t_end), advance t and the step count, or flag the
+// problem done (t_end reached: 1; non-finite dt: 2).
+__device__ inline void schedule_step(nest_prob_state& s, unsigned long long bx, unsigned long long by) {
     s.smax[0] = 0ull;
     s.smax[1] = 0ull;
     if (s.done) return;
@@ -762,6 +748,26 @@ __global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsign
     s.steps += 1;
 }
 
+__global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
+    const int i = blockIdx.x;  // one wave per problem
+    const int lane = threadIdx.x;
+    if (i >= nprob) return;
+    unsigned long long vx = 0, vy = 0;
+    if (slots) {  // fold and clear the stage-2 slots (lane k: slot k)
+        unsigned long long* sl = slots + ((size_t)i * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
+        vx = sl[0];
+        vy = sl[1];
+        sl[0] = 0ull;
+        sl[1] = 0ull;
+    }
+    vx = wave_max_u64(vx);
+    vy = wave_max_u64(vy);
+    if (lane != 0) return;
+    nest_prob_state& s = P[i];
+    // speeds2d / the 1D kernels reduce straight into smax
+    schedule_step(s, s.smax[0] > vx ? s.smax[0] : vx, s.smax[1] > vy ? s.smax[1] : vy);
+}
+
 __global__ __launch_bounds__(64) void reset_speeds_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
@@ -781,7 +787,10 @@ __global__ __launch_bounds__(64) void reset_speeds_kernel(nest_prob_state* P, un
 // 1D (single patch, n interior cells, halo 2, 3 variables)
 // Block of 256 threads owns 256 consecutive interior cells; primitives,
 // slopes and the 257 face fluxes of the block are staged in LDS so each is
-// computed once (benchmark_euler1d.cpp:113-165 numerics).
+// computed once (benchmark_euler1d.cpp:113-165 numerics). Ghost cells are
+// read through the boundary condition (periodic wrap, outflow clamp,
+// reflecting mirror with the momentum negated): the values a ghost fill
+// would leave there, so no fill launch runs between stages.
 // ---------------------------------------------------------------------------
 template <int LIM, int STAGE>
 __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
@@ -794,9 +803,10 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
     const double* in = STAGE == 1 ? a.U : a.U1;
     double* out = STAGE == 1 ? a.U1 : a.U;
     for (int t = threadIdx.x; t < 260; t += 256) {
-        int e = base - 2 + t;
-        e = e < ne ? e : ne - 1;
-        W[t] = prim1d(in[e], in[ne + e], in[2 * ne + e], a.eos);
+        const int g = base - 4 + t;  // interior index (ghosts and the block tail outside [0, n))
+        const int src = nest_map_index(g, n, a.bc) + 2;
+        const double m = in[ne + src];
+        W[t] = prim1d(in[src], a.bc == 2 && (g < 0 || g >= n) ? -m : m, in[2 * ne + src], a.eos);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < 258; t += 256) {  // slopes of cells base-1+t
@@ -831,12 +841,24 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
         for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
     }
     if constexpr (STAGE == 2) {
+        if (a.host_dt > 0.0) return;  // the caller chose dt: no speeds to fold
+        // Device dt: the block's maximum goes to one of NEST_SPEED_SLOTS
+        // slots (spread so that blocks do not queue on one address);
+        // finalize_kernel folds and clears them and schedules the next step.
+        __shared__ unsigned long long wmax[4];
         sm = wave_max_u64(sm);
-        if ((threadIdx.x & 63) == 0) atomicMax(&a.ps->smax[0], sm);
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long b = wmax[0];
+            for (int w = 1; w < 4; ++w) b = b > wmax[w] ? b : wmax[w];
+            atomicMax(a.slots + (size_t)(blockIdx.x & (NEST_SPEED_SLOTS - 1)) * NEST_SLOT_STRIDE, b);
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
+    __shared__ unsigned long long wmax[4];
     const int n = a.n, ne = n + 4;
     const int e = 2 + blockIdx.x * 256 + threadIdx.x;
     unsigned long long sm = 0;
@@ -846,7 +868,13 @@ __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
         sm = (unsigned long long)__double_as_longlong(s);
     }
     sm = wave_max_u64(sm);
-    if ((threadIdx.x & 63) == 0) atomicMax(&a.ps->smax[0], sm);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one atomic per block
+        unsigned long long b = wmax[0];
+        for (int w = 1; w < 4; ++w) b = b > wmax[w] ? b : wmax[w];
+        atomicMax(&a.ps->smax[0], b);
+    }
 }
 
 // periodic (benchmark_euler1d.cpp:102-110), outflow or reflecting (mirror

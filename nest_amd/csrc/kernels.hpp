@@ -78,6 +78,8 @@ struct stage1d_args {
     nest_prob_state* ps;
     nest_eos eos;
     double host_dt;  // > 0: use this dt (ignores the done flag)
+    int bc;          // ghost cells are read through this boundary condition
+    unsigned long long* slots;  // NEST_SPEED_SLOTS speed slots (device dt), zero between launches
 };
 
 NEST_DECLARE_LAUNCHERS(nest_exact)

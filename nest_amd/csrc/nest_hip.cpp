@@ -1081,22 +1081,40 @@ struct nest_solver1d {
     double* U1 = nullptr;
     nest_prob_state* ps_d = nullptr;
     nest_prob_state ps_h{};
+    unsigned long long* slots_d = nullptr;  // stage-2 speed slots (device dt)
     hipStream_t own_stream = nullptr, stream = nullptr;
+    hipGraph_t graph = nullptr;  // kBatch device-dt steps, replayed by run()
+    hipGraphExec_t graph_exec = nullptr;
+    static constexpr int kBatch = 64;
 
     ~nest_solver1d() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         if (U) (void)hipFree(U);
         if (U1) (void)hipFree(U1);
         if (ps_d) (void)hipFree(ps_d);
+        if (slots_d) (void)hipFree(slots_d);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
     const launch_table& L() const { return table(mode); }
-    stage1d_args args(double host_dt) const { return {U, U1, n, ps_d, eos, host_dt}; }
+    stage1d_args args(double host_dt) const { return {U, U1, n, ps_d, eos, host_dt, bc, slots_d}; }
+    // One RK2 step. The stage kernels read ghost cells through the boundary
+    // condition, so no fill runs between them. With host_dt <= 0 the step
+    // takes dt from the device state, and finalize schedules the next one
+    // from the speeds stage 2 leaves in the slots: three launches.
     void step(double host_dt) {
-        const nest_prob_state* flag = host_dt > 0.0 ? nullptr : ps_d;
-        L().fill1d(U, n, bc, flag, stream);
         L().stage1d(1, lim, args(host_dt), stream);
-        L().fill1d(U1, n, bc, flag, stream);
         L().stage1d(2, lim, args(host_dt), stream);
+        if (!(host_dt > 0.0)) L().finalize(ps_d, slots_d, 1, stream);
+    }
+    int ensure_graph() {
+        if (graph_exec) return NEST_OK;
+        NEST_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < kBatch; ++k) step(0.0);
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (e != hipSuccess) return fail(NEST_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        NEST_HIP_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        return NEST_OK;
     }
 };
 
@@ -1120,12 +1138,15 @@ int nest_hip_solver1d_create(int n, double dx, double gamma, int bc, int limiter
     int rc = dev_alloc(&s->U, total);
     if (!rc) rc = dev_alloc(&s->U1, total);
     if (!rc) rc = dev_alloc(&s->ps_d, 1);
+    if (!rc) rc = dev_alloc(&s->slots_d, (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE);
     if (rc) {
         delete s;
         return rc;
     }
     if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipMemset(s->U1, 0, total * sizeof(double)) != hipSuccess ||
+        hipMemset(s->slots_d, 0, (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE * sizeof(unsigned long long)) !=
+            hipSuccess ||
         hipMemcpy(s->ps_d, &s->ps_h, sizeof(nest_prob_state), hipMemcpyHostToDevice) != hipSuccess) {
         delete s;
         return fail(NEST_ERR_HIP, "stream/memory setup failed");
@@ -1176,6 +1197,7 @@ int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt) {
     if (!s) return fail(NEST_ERR_INVALID, "null handle");
     if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
     s->step(dt);
+    s->L().fill1d(s->U, s->n, s->bc, nullptr, s->stream);  // hand out a halo-complete state
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -1188,15 +1210,15 @@ int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* ste
     p.cfl = cfl;
     s->ps_h = p;
     NEST_HIP_TRY(hipMemcpyAsync(s->ps_d, &s->ps_h, sizeof(nest_prob_state), hipMemcpyHostToDevice, s->stream));
-    // halos of the initial state so the first speeds see a consistent array
-    s->L().fill1d(s->U, s->n, s->bc, nullptr, s->stream);
+    // dt of the first step from the initial state; every later one is
+    // scheduled by the step before it
+    s->L().reset_speeds(s->ps_d, s->slots_d, 1, s->stream);
     s->L().speeds1d(s->args(0.0), s->stream);
-    const int batch = 64;
+    s->L().finalize(s->ps_d, nullptr, 1, s->stream);
+    if (int rc = s->ensure_graph()) return rc;
     for (;;) {
-        for (int k = 0; k < batch; ++k) {
-            s->L().finalize(s->ps_d, nullptr, 1, s->stream);
-            s->step(0.0);
-        }
+        // steps past t_end are no-ops (every kernel sees the done flag)
+        NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
         NEST_HIP_TRY(hipGetLastError());
         NEST_HIP_TRY(hipMemcpyAsync(&s->ps_h, s->ps_d, sizeof(nest_prob_state), hipMemcpyDeviceToHost, s->stream));
         NEST_HIP_TRY(hipStreamSynchronize(s->stream));

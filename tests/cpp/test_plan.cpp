@@ -118,7 +118,36 @@ static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int r
     }
 }
 
+// The 1D stage kernel reads extended cell e of the input as interior cell
+// nest_map_index(e - 2, n, bc), momentum negated past a reflecting wall; for
+// every ghost that must be exactly what the oracle's fill leaves there, and
+// cells past the array (a block's tail) must map inside it.
+static void check_1d_gather() {
+    std::mt19937_64 g(11);
+    std::uniform_real_distribution<double> U(-1, 1);
+    for (int n : {2, 3, 5, 17, 256, 1000})
+        for (int bc = 0; bc < 3; ++bc) {
+            const int ne = n + 4;
+            std::vector<double> a(3 * ne);
+            for (auto& x : a) x = U(g);
+            std::vector<double> f = a;
+            nest_oracle::fill_1d(f.data(), n, 2, 3, bc);
+            for (int e = 0; e < ne + 300; ++e) {
+                const int gi = e - 2;
+                const int src = nest_map_index(gi, n, bc) + 2;
+                CHECK(src >= 2 && src < n + 2, "1d gather n%d bc%d e%d in bounds", n, bc, e);
+                if (e >= ne) continue;
+                const bool neg = bc == 2 && (gi < 0 || gi >= n);
+                for (int k = 0; k < 3; ++k) {
+                    const double v = a[k * ne + src];
+                    CHECK(f[k * ne + e] == (k == 1 && neg ? -v : v), "1d gather n%d bc%d e%d k%d", n, bc, e, k);
+                }
+            }
+        }
+}
+
 int main() {
+    check_1d_gather();
     for (int bcx = 0; bcx < 3; ++bcx)  // periodic, outflow, reflecting
         for (int bcy = 0; bcy < 3; ++bcy) {
             check_mesh(96, 80, 3, 2, bcx, bcy, 64);

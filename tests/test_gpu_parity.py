@@ -121,6 +121,30 @@ def test_reference_gpu_test_program_links_and_passes():
     assert len(diffs) == 3 and max(diffs) == 0.0  # bitwise equal to the reference CPU code
 
 
+def test_reference_benchmark_program_on_gpu():
+    """The reference's own 1D benchmark (examples/benchmark/benchmark_euler1d.cpp,
+    the program behind its published 1D GPU figure, N = 1e5), compiled from its
+    source with its GPU section on and linked against libnest_hip.so through
+    the euler1d_gpu_* ABI (oracle/Makefile target _ref/benchmark_euler1d_gpu).
+    Its OpenMP CPU half and our GPU half must take the same number of steps;
+    both rates are printed. t_final = 0.02 instead of the published 0.2 keeps
+    the CPU half short; the per-step rates do not depend on it."""
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref", "benchmark_euler1d_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("reference benchmark program not built (needs the reference tree at build time)")
+    print_case("reference benchmark_euler1d on the GPU, N=1e5 t=0.02")
+    r = subprocess.run([exe, "100000", "0.02"], capture_output=True, text=True, timeout=150)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps = [int(x) for x in re.findall(r"Steps:\s+(\d+)", r.stdout)]
+    rates = [float(x) for x in re.findall(r"Updates/sec:\s+([0-9.eE+-]+)", r.stdout)]
+    assert len(steps) == 2 and steps[0] == steps[1] > 0, steps
+    assert len(rates) == 2 and rates[1] > rates[0], rates
+
+
 # ----------------------------------------------------------------------------
 # 2D mesh
 # ----------------------------------------------------------------------------
