@@ -857,6 +857,123 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
     }
 }
 
+// One whole RK2 step of the 1D solver per launch, out of place: U^{n+1}
+// (into a.U1) from U^n (a.U). A block owns 256 output cells g0 .. g0+255 and
+// stages U^n over g0-4 .. g0+259 (ghosts read through the boundary
+// condition) in LDS; it computes stage 1 at g0-2 .. g0+257, so U1 never
+// leaves the block, then stage 2 at its own cells. The numerics and their
+// order are those of the two stage kernels, cell for cell: stage 1 at a
+// position past a periodic end sees the stencil of the cell it wraps to, so
+// it equals that cell's U1 bit for bit; past an outflow or reflecting wall
+// U1 is set from the block's own boundary cells, as the ghost fill of U1
+// would set it. The block's maximum wave speed of U^{n+1} goes to a speed
+// slot (device dt) for finalize_kernel.
+template <int LIM>
+__global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
+    constexpr int T = 264;  // tile position t holds interior cell g0 - 4 + t
+    __shared__ double C[3][T];  // conserved U^n, then U1 at t in [2, 262)
+    __shared__ prim4 W[T];
+    __shared__ prim4 S[T];
+    __shared__ double F[3][T];  // F[k][t]: face between tile positions t and t + 1
+    __shared__ unsigned long long wmax[4];
+    if (!(a.host_dt > 0.0) && a.ps->done) return;
+    const int n = a.n, ne = n + 4;
+    const int g0 = blockIdx.x * 256;
+    const double* in = a.U;
+    double* out = a.U1;
+    const double dt = a.host_dt > 0.0 ? a.host_dt : a.ps->dt;
+    const double dtdx = dt / a.ps->dx;
+    for (int t = threadIdx.x; t < T; t += 256) {
+        const int g = g0 - 4 + t;
+        const int src = nest_map_index(g, n, a.bc) + 2;
+        const double m = in[ne + src];
+        C[0][t] = in[src];
+        C[1][t] = a.bc == 2 && (g < 0 || g >= n) ? -m : m;
+        C[2][t] = in[2 * ne + src];
+        W[t] = prim1d(C[0][t], C[1][t], C[2][t], a.eos);
+    }
+    __syncthreads();
+    // ---- stage 1 at t in [2, 262)
+    for (int t = threadIdx.x + 1; t < T - 1; t += 256) S[t] = slope4<LIM>(sub4(W[t], W[t - 1]), sub4(W[t + 1], W[t]));
+    __syncthreads();
+    for (int t = threadIdx.x + 1; t < T - 2; t += 256) {
+        double f[4];
+        face_flux<false>(face_plus(W[t], S[t]), face_minus(W[t + 1], S[t + 1]), a.eos, f);
+        F[0][t] = f[0];
+        F[1][t] = f[1];
+        F[2][t] = f[2];
+    }
+    __syncthreads();
+    double u1[2][3];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int t = threadIdx.x + 2 + 256 * r;
+        if (t < T - 2) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) u1[r][k] = C[k][t] - dtdx * (F[k][t] - F[k][t - 1]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int t = threadIdx.x + 2 + 256 * r;
+        if (t < T - 2) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) C[k][t] = u1[r][k];
+        }
+    }
+    __syncthreads();
+    if (a.bc != 0) {  // U1 past an outflow / reflecting wall, from the block's own cells
+        for (int t = threadIdx.x + 2; t < T - 2; t += 256) {
+            const int g = g0 - 4 + t;
+            if (g >= 0 && g < n) continue;
+            const int ts = nest_map_index(g, n, a.bc) - (g0 - 4);
+            const bool neg = a.bc == 2;
+            C[0][t] = C[0][ts];
+            C[1][t] = neg ? -C[1][ts] : C[1][ts];
+            C[2][t] = C[2][ts];
+        }
+        __syncthreads();
+    }
+    // ---- stage 2 at t in [4, 260)
+    for (int t = threadIdx.x + 2; t < T - 2; t += 256) W[t] = prim1d(C[0][t], C[1][t], C[2][t], a.eos);
+    __syncthreads();
+    for (int t = threadIdx.x + 3; t < T - 3; t += 256) S[t] = slope4<LIM>(sub4(W[t], W[t - 1]), sub4(W[t + 1], W[t]));
+    __syncthreads();
+    for (int t = threadIdx.x + 3; t < T - 4; t += 256) {
+        double f[4];
+        face_flux<false>(face_plus(W[t], S[t]), face_minus(W[t + 1], S[t + 1]), a.eos, f);
+        F[0][t] = f[0];
+        F[1][t] = f[1];
+        F[2][t] = f[2];
+    }
+    __syncthreads();
+    const int g = g0 + threadIdx.x;
+    unsigned long long sm = 0;
+    if (g < n) {
+        const int t = threadIdx.x + 4, e = g + 2;
+        double o[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[k] = C[k][t] - dtdx * (F[k][t] - F[k][t - 1]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[k] = 0.5 * (in[k * ne + e] + o[k]);
+        prim4 w = prim1d(o[0], o[1], o[2], a.eos);
+        double sp = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
+        sm = (unsigned long long)__double_as_longlong(sp);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
+    }
+    if (a.host_dt > 0.0) return;  // the caller chose dt: no speeds to fold
+    sm = wave_max_u64(sm);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmax[0];
+        for (int w = 1; w < 4; ++w) b = b > wmax[w] ? b : wmax[w];
+        atomicMax(a.slots + (size_t)(blockIdx.x & (NEST_SPEED_SLOTS - 1)) * NEST_SLOT_STRIDE, b);
+    }
+}
+
 __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
     __shared__ unsigned long long wmax[4];
     const int n = a.n, ne = n + 4;
@@ -964,6 +1081,15 @@ void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s) {
     const int grid = (a.n + 255) / 256;
     if (stage == 1) stage1d_dispatch<1>(lim, a, grid, s);
     else stage1d_dispatch<2>(lim, a, grid, s);
+}
+
+void launch_step1d(int lim, const stage1d_args& a, hipStream_t s) {
+    const int grid = (a.n + 255) / 256;
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((step1d_kernel<LIM_MC>), grid, 256, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((step1d_kernel<LIM_NONE>), grid, 256, 0, s, a); break;
+        default: hipLaunchKernelGGL((step1d_kernel<LIM_MINMOD>), grid, 256, 0, s, a); break;
+    }
 }
 
 void launch_speeds1d(const stage1d_args& a, hipStream_t s) {

@@ -15,6 +15,7 @@
     void launch_finalize(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s);  \
     void launch_reset_speeds(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s); \
     void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s);                  \
+    void launch_step1d(int lim, const stage1d_args& a, hipStream_t s);                              \
     void launch_speeds1d(const stage1d_args& a, hipStream_t s);                                     \
     void launch_fill1d(double* U, int n, int bc, const nest_prob_state* ps, hipStream_t s);         \
     void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s);              \

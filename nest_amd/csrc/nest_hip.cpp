@@ -48,6 +48,7 @@ struct launch_table {
     void (*finalize)(nest_prob_state*, unsigned long long*, int, hipStream_t);
     void (*reset_speeds)(nest_prob_state*, unsigned long long*, int, hipStream_t);
     void (*stage1d)(int, int, const stage1d_args&, hipStream_t);
+    void (*step1d)(int, const stage1d_args&, hipStream_t);
     void (*speeds1d)(const stage1d_args&, hipStream_t);
     void (*fill1d)(double*, int, int, const nest_prob_state*, hipStream_t);
     void (*adj_y)(int, const adj_args&, int, int, hipStream_t);
@@ -59,13 +60,15 @@ struct launch_table {
 const launch_table kExact = {nest_exact::launch_stage2d,     nest_exact::launch_speeds2d,
                              nest_exact::launch_fill_regions, nest_exact::launch_finalize,
                              nest_exact::launch_reset_speeds, nest_exact::launch_stage1d,
-                             nest_exact::launch_speeds1d,     nest_exact::launch_fill1d,
+                             nest_exact::launch_step1d,       nest_exact::launch_speeds1d,
+                             nest_exact::launch_fill1d,
                              nest_exact::launch_adj_y,        nest_exact::launch_adj_x,
                              nest_exact::launch_pack};
 const launch_table kFast = {nest_fast::launch_stage2d,     nest_fast::launch_speeds2d,
                             nest_fast::launch_fill_regions, nest_fast::launch_finalize,
                             nest_fast::launch_reset_speeds, nest_fast::launch_stage1d,
-                            nest_fast::launch_speeds1d,     nest_fast::launch_fill1d,
+                            nest_fast::launch_step1d,       nest_fast::launch_speeds1d,
+                            nest_fast::launch_fill1d,
                             nest_fast::launch_adj_y,        nest_fast::launch_adj_x,
                             nest_fast::launch_pack};
 
@@ -1098,19 +1101,26 @@ struct nest_solver1d {
     }
     const launch_table& L() const { return table(mode); }
     stage1d_args args(double host_dt) const { return {U, U1, n, ps_d, eos, host_dt, bc, slots_d}; }
-    // One RK2 step. The stage kernels read ghost cells through the boundary
-    // condition, so no fill runs between them. With host_dt <= 0 the step
-    // takes dt from the device state, and finalize schedules the next one
-    // from the speeds stage 2 leaves in the slots: three launches.
+    // One RK2 step with the caller's dt, in place: the two stage kernels
+    // (they read ghost cells through the boundary condition, so no fill runs
+    // between them).
     void step(double host_dt) {
         L().stage1d(1, lim, args(host_dt), stream);
         L().stage1d(2, lim, args(host_dt), stream);
-        if (!(host_dt > 0.0)) L().finalize(ps_d, slots_d, 1, stream);
+    }
+    // One device-dt RK2 step, out of place: the fused step kernel from the
+    // state in U (even k) or U1 (odd k) into the other, then finalize
+    // schedules the next step from the speeds the step left in the slots.
+    void device_step(int k) {
+        stage1d_args a = args(0.0);
+        if (k & 1) std::swap(a.U, a.U1);
+        L().step1d(lim, a, stream);
+        L().finalize(ps_d, slots_d, 1, stream);
     }
     int ensure_graph() {
         if (graph_exec) return NEST_OK;
         NEST_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < kBatch; ++k) step(0.0);
+        for (int k = 0; k < kBatch; ++k) device_step(k);  // kBatch is even: the state ends in U
         hipError_t e = hipStreamEndCapture(stream, &graph);
         if (e != hipSuccess) return fail(NEST_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         NEST_HIP_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
@@ -1224,6 +1234,10 @@ int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* ste
         NEST_HIP_TRY(hipStreamSynchronize(s->stream));
         if (s->ps_h.done) break;
     }
+    // steps alternate U -> U1 -> U: after an odd number of them the state is in U1
+    if (s->ps_h.steps & 1)
+        NEST_HIP_TRY(hipMemcpyAsync(s->U, s->U1, 3 * (size_t)(s->n + 4) * sizeof(double), hipMemcpyDeviceToDevice,
+                                    s->stream));
     s->L().fill1d(s->U, s->n, s->bc, nullptr, s->stream);
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
     if (steps) *steps = s->ps_h.steps;

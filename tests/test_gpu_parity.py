@@ -74,6 +74,35 @@ def test_1d_sod_run_to_t_end(oracle, mode):
     assert abs(got[0].sum() - want[0].sum()) <= 1e-12 * want[0].sum()
 
 
+@pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
+def test_1d_run_fused_step_sizes(oracle, bc):
+    """Solver1D.run takes each device-dt step in one fused launch that keeps
+    U1 in LDS, alternating the state between two buffers. Bitwise equal to
+    the oracle's run (exact numerics) for sizes of 2 and 3 cells (one block
+    whose halo wraps / mirrors / clamps onto itself), 257 (a last block of one
+    cell) and 1000, each at two end times whose step counts differ in parity
+    (the state ends in either buffer)."""
+    cases = [(2, (0.5, 0.62)), (3, (0.62, 0.7)), (257, (0.05, 0.0503)), (1000, (0.05, 0.0501))]
+    for n, t_ends in cases:
+        parities = set()
+        for t_end in t_ends:
+            print_case("1d run fused", n, bc, t_end)
+            U = P.sod1d(n)
+            oracle.fill_1d(U, n, bc=bc)
+            ref = U.copy()
+            k_ref, _ = oracle.run_1d(ref, n, t_end, 0.4, 1.0 / n, bc=bc)
+            s = Solver1D(n, 1.0 / n, bc=bc, mode="exact")
+            s.upload(U)
+            k = s.run(t_end, 0.4)
+            got = s.download()
+            s.close()
+            assert k == k_ref, (n, t_end, k, k_ref)
+            assert np.array_equal(got.reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2]), (n, t_end)
+            assert np.array_equal(got, ref), "ghost cells of the handed-out state"
+            parities.add(k & 1)
+        assert parities == {0, 1}, (n, bc)
+
+
 def test_1d_reference_abi_periodic_sod(oracle):
     """Reference GPU-vs-CPU case (tests/test_euler_cuda.cpp:27-43): periodic
     Sod N=100, t=0.1, CFL 0.3 through the reference-compatible euler1d_gpu_*
