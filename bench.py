@@ -267,13 +267,13 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
             "fd_max_rel_err_per_direction": worst}
 
 
-def oned_leg(large_steps: int = 200):
+def oned_leg(large_steps: int = 200, sod_t_end: float = 0.2):
     """1D Sod with device dt through Solver1D.run (two launches per step,
     replayed 64 steps at a time from a captured graph):
     * the configuration of the reference's published 1D GPU figure
       (examples/benchmark/benchmark_euler1d.cpp: N = 1e5, periodic, CFL 0.4,
       t = 0.2; exact numerics, as the euler1d_gpu_* drop-in), median of 3 runs
-      after one untimed run;
+      after one untimed run (`sod_t_end` shortens it for profiling runs);
     * N = 2^24 cells (outflow, fast numerics), about `large_steps` steps: the
       stage kernels' bandwidth, 120 B per cell-step (stage 1 reads and writes
       3 doubles, stage 2 reads 6 and writes 3)."""
@@ -286,11 +286,11 @@ def oned_leg(large_steps: int = 200):
     for r in range(4):
         s.upload(U)
         t0 = time.perf_counter()
-        steps = s.run(0.2, 0.4)
+        steps = s.run(sod_t_end, 0.4)
         times.append(time.perf_counter() - t0)
     s.close()
     t = sorted(times[1:])[1]
-    out["sod_1e5"] = {"cells": n, "t_end": 0.2, "bc": "periodic", "numerics": "exact", "steps": steps,
+    out["sod_1e5"] = {"cells": n, "t_end": sod_t_end, "bc": "periodic", "numerics": "exact", "steps": steps,
                       "ms": round(t * 1e3, 2), "us_per_step": round(t * 1e6 / steps, 2),
                       "mcups": round(n * 2.0 * steps / t / 1e6, 1),
                       "reference_published_mcups": 5000.0}
@@ -420,6 +420,7 @@ def main():
                     help="central-difference directions of the adjoint gradient check; 0 skips the check")
     ap.add_argument("--kh-steps", type=int, default=100, help="KH 8192^2 + 64 batched 256^2 adjoints leg; 0 skips it")
     ap.add_argument("--oned-steps", type=int, default=200, help="1D leg (N=1e5 to t=0.2; N=2^24 this many steps); 0 skips it")
+    ap.add_argument("--oned-sod-t-end", type=float, default=0.2, help="end time of the 1D N=1e5 run (profiling aid)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling aid: one plain device copy of this many MiB after the legs")
     args = ap.parse_args()
@@ -532,7 +533,7 @@ def main():
     oned = None
     if args.oned_steps > 0:
         log("bench: 1D Sod (N=1e5 to t=0.2; N=2^24)")
-        oned = oned_leg(args.oned_steps)
+        oned = oned_leg(args.oned_steps, args.oned_sod_t_end)
 
     cpu = None
     if not args.no_cpu:

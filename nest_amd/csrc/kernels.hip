@@ -871,10 +871,19 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
 template <int LIM>
 __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     constexpr int T = 264;  // tile position t holds interior cell g0 - 4 + t
+    // one array per variable (8-byte lane stride: no LDS bank conflicts);
+    // the passive component of prim4 is zero in 1D
     __shared__ double C[3][T];  // conserved U^n, then U1 at t in [2, 262)
-    __shared__ prim4 W[T];
-    __shared__ prim4 S[T];
+    __shared__ double Wv[3][T];  // primitives rho, u, p
+    __shared__ double Sv[3][T];  // their limited slopes
     __shared__ double F[3][T];  // F[k][t]: face between tile positions t and t + 1
+    auto W = [&](int t) { return prim4{Wv[0][t], Wv[1][t], Wv[2][t], 0.0}; };
+    auto S = [&](int t) { return prim4{Sv[0][t], Sv[1][t], Sv[2][t], 0.0}; };
+    auto put = [](double (*A)[T], int t, const prim4& w) {
+        A[0][t] = w.r;
+        A[1][t] = w.u;
+        A[2][t] = w.p;
+    };
     __shared__ unsigned long long wmax[4];
     if (!(a.host_dt > 0.0) && a.ps->done) return;
     const int n = a.n, ne = n + 4;
@@ -890,15 +899,15 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
         C[0][t] = in[src];
         C[1][t] = a.bc == 2 && (g < 0 || g >= n) ? -m : m;
         C[2][t] = in[2 * ne + src];
-        W[t] = prim1d(C[0][t], C[1][t], C[2][t], a.eos);
+        put(Wv, t, prim1d(C[0][t], C[1][t], C[2][t], a.eos));
     }
     __syncthreads();
     // ---- stage 1 at t in [2, 262)
-    for (int t = threadIdx.x + 1; t < T - 1; t += 256) S[t] = slope4<LIM>(sub4(W[t], W[t - 1]), sub4(W[t + 1], W[t]));
+    for (int t = threadIdx.x + 1; t < T - 1; t += 256) put(Sv, t, slope4<LIM>(sub4(W(t), W(t - 1)), sub4(W(t + 1), W(t))));
     __syncthreads();
     for (int t = threadIdx.x + 1; t < T - 2; t += 256) {
         double f[4];
-        face_flux<false>(face_plus(W[t], S[t]), face_minus(W[t + 1], S[t + 1]), a.eos, f);
+        face_flux<false>(face_plus(W(t), S(t)), face_minus(W(t + 1), S(t + 1)), a.eos, f);
         F[0][t] = f[0];
         F[1][t] = f[1];
         F[2][t] = f[2];
@@ -936,13 +945,13 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
         __syncthreads();
     }
     // ---- stage 2 at t in [4, 260)
-    for (int t = threadIdx.x + 2; t < T - 2; t += 256) W[t] = prim1d(C[0][t], C[1][t], C[2][t], a.eos);
+    for (int t = threadIdx.x + 2; t < T - 2; t += 256) put(Wv, t, prim1d(C[0][t], C[1][t], C[2][t], a.eos));
     __syncthreads();
-    for (int t = threadIdx.x + 3; t < T - 3; t += 256) S[t] = slope4<LIM>(sub4(W[t], W[t - 1]), sub4(W[t + 1], W[t]));
+    for (int t = threadIdx.x + 3; t < T - 3; t += 256) put(Sv, t, slope4<LIM>(sub4(W(t), W(t - 1)), sub4(W(t + 1), W(t))));
     __syncthreads();
     for (int t = threadIdx.x + 3; t < T - 4; t += 256) {
         double f[4];
-        face_flux<false>(face_plus(W[t], S[t]), face_minus(W[t + 1], S[t + 1]), a.eos, f);
+        face_flux<false>(face_plus(W(t), S(t)), face_minus(W(t + 1), S(t + 1)), a.eos, f);
         F[0][t] = f[0];
         F[1][t] = f[1];
         F[2][t] = f[2];
