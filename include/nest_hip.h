@@ -79,11 +79,17 @@ int nest_hip_solver1d_download(nest_solver1d* s, double* host);
 /* dt = cfl*dx/max(|u|+c) of the current state; blocking
    (euler1d_solver.hpp:58, euler1d_solver.cu:100-119). */
 int nest_hip_solver1d_compute_dt(nest_solver1d* s, double cfl, double* dt);
-/* One SSP-RK2 step with the given dt (euler1d_solver.hpp:66). */
+/* One SSP-RK2 step with the given dt (euler1d_solver.hpp:66): two stage
+   launches that read ghost cells through the boundary condition, then a
+   ghost fill so the state handed out is halo-complete. */
 int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt);
 /* Run to t_final with dt = min(cfl*dx/max(|u|+c), t_final - t) computed on
-   the device every step; the host only polls completion every batch of
-   steps (euler1d_solver.hpp:76). *steps receives the step count. */
+   the device every step (euler1d_solver.hpp:76). Each step is one fused
+   launch (U1 kept on chip, the state alternating between the handle's two
+   buffers) and one dt launch; steps are replayed 64 at a time from a graph
+   captured on the first call, and the host polls completion once per 64.
+   The final state is in the handle's state buffer, halo-complete. *steps
+   receives the step count. */
 int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* steps);
 /* Use the caller's stream (hipStream_t, NULL = handle's own). */
 int nest_hip_solver1d_set_stream(nest_solver1d* s, void* stream);
