@@ -85,9 +85,10 @@ int nest_hip_solver1d_compute_dt(nest_solver1d* s, double cfl, double* dt);
 int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt);
 /* Run to t_final with dt = min(cfl*dx/max(|u|+c), t_final - t) computed on
    the device every step (euler1d_solver.hpp:76). Each step is one fused
-   launch (U1 kept on chip, the state alternating between the handle's two
-   buffers) and one dt launch; steps are replayed 64 at a time from a graph
-   captured on the first call, and the host polls completion once per 64.
+   launch (dt from the previous step's wave speeds, U1 kept on chip, the
+   state alternating between the handle's two buffers); steps are replayed
+   48 at a time from a graph captured on the first call, and the host polls
+   completion once per 48.
    The final state is in the handle's state buffer, halo-complete. *steps
    receives the step count. */
 int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* steps);

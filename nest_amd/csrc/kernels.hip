@@ -841,7 +841,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
         for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
     }
     if constexpr (STAGE == 2) {
-        if (a.host_dt > 0.0) return;  // the caller chose dt: no speeds to fold
+        if (a.host_dt > 0.0 || !a.slots) return;  // the caller chose dt: no speeds to fold
         // Device dt: the block's maximum goes to one of NEST_SPEED_SLOTS
         // slots (spread so that blocks do not queue on one address);
         // finalize_kernel folds and clears them and schedules the next step.
@@ -866,8 +866,10 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
 // position past a periodic end sees the stencil of the cell it wraps to, so
 // it equals that cell's U1 bit for bit; past an outflow or reflecting wall
 // U1 is set from the block's own boundary cells, as the ghost fill of U1
-// would set it. The block's maximum wave speed of U^{n+1} goes to a speed
-// slot (device dt) for finalize_kernel.
+// would set it. Every block takes the step's dt from the speeds the
+// previous step left in the slots (the arithmetic of schedule_step), block 0
+// records it, and the block's maximum wave speed of U^{n+1} goes to the next
+// slot set: one launch per step.
 template <int LIM>
 __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     constexpr int T = 264;  // tile position t holds interior cell g0 - 4 + t
@@ -885,13 +887,20 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
         A[2][t] = w.p;
     };
     __shared__ unsigned long long wmax[4];
-    if (!(a.host_dt > 0.0) && a.ps->done) return;
+    __shared__ double s_dt;
+    __shared__ int s_go;
     const int n = a.n, ne = n + 4;
     const int g0 = blockIdx.x * 256;
     const double* in = a.U;
     double* out = a.U1;
-    const double dt = a.host_dt > 0.0 ? a.host_dt : a.ps->dt;
-    const double dtdx = dt / a.ps->dx;
+    // Issued ahead of the tile loads, used after them: the run's flag, this
+    // step's time and the previous step's speeds. A finished or failed run
+    // makes every later step a no-op. (Block 0 of this launch may set the
+    // flag while others read it; it does so only when every block finds the
+    // same from the same inputs.)
+    const int done_in = a.run->done;
+    const double t_in = a.run->t[a.par];
+    const unsigned long long v_in = threadIdx.x < 64 ? a.slots_rd[(size_t)threadIdx.x * NEST_SLOT_STRIDE] : 0ull;
     for (int t = threadIdx.x; t < T; t += 256) {
         const int g = g0 - 4 + t;
         const int src = nest_map_index(g, n, a.bc) + 2;
@@ -901,7 +910,43 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
         C[2][t] = in[2 * ne + src];
         put(Wv, t, prim1d(C[0][t], C[1][t], C[2][t], a.eos));
     }
+    // the step's dt (its read overlaps the tile loads above)
+    if (threadIdx.x < 64) {
+        const unsigned long long v = wave_max_u64(v_in);
+        if (threadIdx.x == 0) {
+            const nest_prob_state& P = *a.ps;
+            const double t = t_in;
+            const double sx = __longlong_as_double((long long)v);
+            int done = 0;
+            double dtn = 0.0;
+            if (done_in) {
+                done = -1;  // already finished: leave the run's state alone
+            } else if (!(t < P.t_end)) {
+                done = 1;
+            } else {
+                dtn = P.fixed_dt > 0.0 ? P.fixed_dt : P.cfl * P.dx / sx;
+                dtn = dmin(dtn, P.t_end - t);
+                if (!(dtn > 0.0) || dtn != dtn || dtn == __builtin_inf() || sx != sx) done = 2;
+            }
+            s_go = !done;
+            s_dt = dtn;
+            if (blockIdx.x == 0 && done >= 0) {
+                if (done) {
+                    a.run->done = done;
+                    if (done == 2) a.run->dt = dtn;
+                } else {
+                    a.run->dt = dtn;
+                    a.run->t[a.par ^ 1] = t + dtn;
+                    a.run->steps += 1;
+                }
+            }
+        }
+        if (blockIdx.x == 0) a.slots_clr[(size_t)threadIdx.x * NEST_SLOT_STRIDE] = 0ull;  // for the next step
+    }
     __syncthreads();
+    if (!s_go) return;
+    const double dt = s_dt;
+    const double dtdx = dt / a.ps->dx;
     // ---- stage 1 at t in [2, 262)
     for (int t = threadIdx.x + 1; t < T - 1; t += 256) put(Sv, t, slope4<LIM>(sub4(W(t), W(t - 1)), sub4(W(t + 1), W(t))));
     __syncthreads();
@@ -972,7 +1017,6 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
     }
-    if (a.host_dt > 0.0) return;  // the caller chose dt: no speeds to fold
     sm = wave_max_u64(sm);
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
     __syncthreads();
@@ -996,10 +1040,10 @@ __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
     sm = wave_max_u64(sm);
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
     __syncthreads();
-    if (threadIdx.x == 0) {  // one atomic per block
+    if (threadIdx.x == 0) {  // one atomic per block: into slot 0 of a slot set, or smax
         unsigned long long b = wmax[0];
         for (int w = 1; w < 4; ++w) b = b > wmax[w] ? b : wmax[w];
-        atomicMax(&a.ps->smax[0], b);
+        atomicMax(a.slots ? a.slots : &a.ps->smax[0], b);
     }
 }
 

@@ -80,7 +80,14 @@ struct stage1d_args {
     nest_eos eos;
     double host_dt;  // > 0: use this dt (ignores the done flag)
     int bc;          // ghost cells are read through this boundary condition
-    unsigned long long* slots;  // NEST_SPEED_SLOTS speed slots (device dt), zero between launches
+    unsigned long long* slots;  // NEST_SPEED_SLOTS speed slots the launch folds its speeds into (device dt)
+    // fused 1D device-dt step only: the time-stepping state, the slot set the
+    // previous step wrote (this step's dt), the set to clear for the next
+    // step, and the parity of the step
+    nest_run1d_state* run;
+    const unsigned long long* slots_rd;
+    unsigned long long* slots_clr;
+    int par;
 };
 
 NEST_DECLARE_LAUNCHERS(nest_exact)

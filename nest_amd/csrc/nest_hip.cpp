@@ -1084,11 +1084,14 @@ struct nest_solver1d {
     double* U1 = nullptr;
     nest_prob_state* ps_d = nullptr;
     nest_prob_state ps_h{};
-    unsigned long long* slots_d = nullptr;  // stage-2 speed slots (device dt)
+    unsigned long long* slots_d = nullptr;  // 3 rotating sets of speed slots (device dt)
+    nest_run1d_state* run_d = nullptr;      // time-stepping state of run()
+    nest_run1d_state run_h{};
     hipStream_t own_stream = nullptr, stream = nullptr;
     hipGraph_t graph = nullptr;  // kBatch device-dt steps, replayed by run()
     hipGraphExec_t graph_exec = nullptr;
-    static constexpr int kBatch = 64;
+    static constexpr int kBatch = 48;  // a multiple of 2 (buffers, time slots) and 3 (slot sets)
+    static constexpr size_t kSet = (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE;
 
     ~nest_solver1d() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1097,10 +1100,13 @@ struct nest_solver1d {
         if (U1) (void)hipFree(U1);
         if (ps_d) (void)hipFree(ps_d);
         if (slots_d) (void)hipFree(slots_d);
+        if (run_d) (void)hipFree(run_d);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
     const launch_table& L() const { return table(mode); }
-    stage1d_args args(double host_dt) const { return {U, U1, n, ps_d, eos, host_dt, bc, slots_d}; }
+    stage1d_args args(double host_dt) const {
+        return {U, U1, n, ps_d, eos, host_dt, bc, nullptr, nullptr, nullptr, nullptr, 0};
+    }
     // One RK2 step with the caller's dt, in place: the two stage kernels
     // (they read ghost cells through the boundary condition, so no fill runs
     // between them).
@@ -1108,14 +1114,19 @@ struct nest_solver1d {
         L().stage1d(1, lim, args(host_dt), stream);
         L().stage1d(2, lim, args(host_dt), stream);
     }
-    // One device-dt RK2 step, out of place: the fused step kernel from the
-    // state in U (even k) or U1 (odd k) into the other, then finalize
-    // schedules the next step from the speeds the step left in the slots.
+    // Device-dt RK2 step k, out of place, one launch: the fused step kernel
+    // from the state in U (even k) or U1 (odd k) into the other. It takes dt
+    // from slot set (k + 2) % 3 (written by step k - 1), folds its speeds into
+    // set k % 3 and clears set (k + 1) % 3 for step k + 1.
     void device_step(int k) {
         stage1d_args a = args(0.0);
         if (k & 1) std::swap(a.U, a.U1);
+        a.par = k & 1;
+        a.run = run_d;
+        a.slots = slots_d + (size_t)(k % 3) * kSet;
+        a.slots_rd = slots_d + (size_t)((k + 2) % 3) * kSet;
+        a.slots_clr = slots_d + (size_t)((k + 1) % 3) * kSet;
         L().step1d(lim, a, stream);
-        L().finalize(ps_d, slots_d, 1, stream);
     }
     int ensure_graph() {
         if (graph_exec) return NEST_OK;
@@ -1148,15 +1159,15 @@ int nest_hip_solver1d_create(int n, double dx, double gamma, int bc, int limiter
     int rc = dev_alloc(&s->U, total);
     if (!rc) rc = dev_alloc(&s->U1, total);
     if (!rc) rc = dev_alloc(&s->ps_d, 1);
-    if (!rc) rc = dev_alloc(&s->slots_d, (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE);
+    if (!rc) rc = dev_alloc(&s->slots_d, 3 * nest_solver1d::kSet);
+    if (!rc) rc = dev_alloc(&s->run_d, 1);
     if (rc) {
         delete s;
         return rc;
     }
     if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipMemset(s->U1, 0, total * sizeof(double)) != hipSuccess ||
-        hipMemset(s->slots_d, 0, (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE * sizeof(unsigned long long)) !=
-            hipSuccess ||
+        hipMemset(s->slots_d, 0, 3 * nest_solver1d::kSet * sizeof(unsigned long long)) != hipSuccess ||
         hipMemcpy(s->ps_d, &s->ps_h, sizeof(nest_prob_state), hipMemcpyHostToDevice) != hipSuccess) {
         delete s;
         return fail(NEST_ERR_HIP, "stream/memory setup failed");
@@ -1220,20 +1231,27 @@ int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* ste
     p.cfl = cfl;
     s->ps_h = p;
     NEST_HIP_TRY(hipMemcpyAsync(s->ps_d, &s->ps_h, sizeof(nest_prob_state), hipMemcpyHostToDevice, s->stream));
-    // dt of the first step from the initial state; every later one is
-    // scheduled by the step before it
-    s->L().reset_speeds(s->ps_d, s->slots_d, 1, s->stream);
-    s->L().speeds1d(s->args(0.0), s->stream);
-    s->L().finalize(s->ps_d, nullptr, 1, s->stream);
+    s->run_h = nest_run1d_state{};
+    NEST_HIP_TRY(hipMemcpyAsync(s->run_d, &s->run_h, sizeof(nest_run1d_state), hipMemcpyHostToDevice, s->stream));
+    // the speeds of the initial state go to slot set 2, which the first step
+    // reads its dt from; sets 0 and 1 start clear
+    for (int set = 0; set < 3; ++set)
+        s->L().reset_speeds(s->ps_d, s->slots_d + (size_t)set * nest_solver1d::kSet, 1, s->stream);
+    stage1d_args a0 = s->args(0.0);
+    a0.slots = s->slots_d + 2 * nest_solver1d::kSet;
+    s->L().speeds1d(a0, s->stream);
     if (int rc = s->ensure_graph()) return rc;
     for (;;) {
-        // steps past t_end are no-ops (every kernel sees the done flag)
+        // steps past t_end are no-ops (every launch sees the done flag)
         NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
         NEST_HIP_TRY(hipGetLastError());
-        NEST_HIP_TRY(hipMemcpyAsync(&s->ps_h, s->ps_d, sizeof(nest_prob_state), hipMemcpyDeviceToHost, s->stream));
+        NEST_HIP_TRY(hipMemcpyAsync(&s->run_h, s->run_d, sizeof(nest_run1d_state), hipMemcpyDeviceToHost, s->stream));
         NEST_HIP_TRY(hipStreamSynchronize(s->stream));
-        if (s->ps_h.done) break;
+        if (s->run_h.done) break;
     }
+    s->ps_h.steps = s->run_h.steps;
+    s->ps_h.done = s->run_h.done;
+    s->ps_h.t = s->run_h.t[s->run_h.steps & 1];
     // steps alternate U -> U1 -> U: after an odd number of them the state is in U1
     if (s->ps_h.steps & 1)
         NEST_HIP_TRY(hipMemcpyAsync(s->U, s->U1, 3 * (size_t)(s->n + 4) * sizeof(double), hipMemcpyDeviceToDevice,

@@ -58,6 +58,17 @@ struct nest_prob_state {
     int pad;
 };
 
+// Time-stepping state of the 1D solver's fused device-dt loop. Step k reads
+// the time of slot k & 1 and block 0 of its launch writes slot (k + 1) & 1,
+// so no block of a launch reads what another block of it writes. steps counts
+// the steps taken; done is 1 once t reached t_end, 2 on a non-finite dt.
+struct nest_run1d_state {
+    double t[2];
+    double dt;
+    int steps;
+    int done;
+};
+
 // One halo region of the device exchange plan: copies an ni x nj box of all
 // variables from a source patch array into the ghost cells of a destination
 // patch array. Coordinates are extended-space local indices. A broadcast flag
