@@ -268,8 +268,8 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
 
 
 def oned_leg(large_steps: int = 200, sod_t_end: float = 0.2):
-    """1D Sod with device dt through Solver1D.run (two launches per step,
-    replayed 64 steps at a time from a captured graph):
+    """1D Sod with device dt through Solver1D.run (one launch per step, dt
+    included, replayed 48 steps at a time from a captured graph):
     * the configuration of the reference's published 1D GPU figure
       (examples/benchmark/benchmark_euler1d.cpp: N = 1e5, periodic, CFL 0.4,
       t = 0.2; exact numerics, as the euler1d_gpu_* drop-in), median of 3 runs
