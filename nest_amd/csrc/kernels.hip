@@ -870,6 +870,12 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
 // previous step left in the slots (the arithmetic of schedule_step), block 0
 // records it, and the block's maximum wave speed of U^{n+1} goes to the next
 // slot set: one launch per step.
+// Speed slots the fused 1D step uses of each set (a power of two, at most
+// NEST_SPEED_SLOTS): every block reads all of them for its dt, so fewer lines
+// to read, while the blocks' atomics stay spread over this many addresses.
+constexpr int kStepSlots = 16;
+static_assert((kStepSlots & (kStepSlots - 1)) == 0 && kStepSlots <= NEST_SPEED_SLOTS, "slot count");
+
 template <int LIM>
 __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     constexpr int T = 264;  // tile position t holds interior cell g0 - 4 + t
@@ -900,7 +906,8 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     // same from the same inputs.)
     const int done_in = a.run->done;
     const double t_in = a.run->t[a.par];
-    const unsigned long long v_in = threadIdx.x < 64 ? a.slots_rd[(size_t)threadIdx.x * NEST_SLOT_STRIDE] : 0ull;
+    const unsigned long long v_in =
+        threadIdx.x < kStepSlots ? a.slots_rd[(size_t)threadIdx.x * NEST_SLOT_STRIDE] : 0ull;
     for (int t = threadIdx.x; t < T; t += 256) {
         const int g = g0 - 4 + t;
         const int src = nest_map_index(g, n, a.bc) + 2;
@@ -941,7 +948,8 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
                 }
             }
         }
-        if (blockIdx.x == 0) a.slots_clr[(size_t)threadIdx.x * NEST_SLOT_STRIDE] = 0ull;  // for the next step
+        if (blockIdx.x == 0 && threadIdx.x < kStepSlots)
+            a.slots_clr[(size_t)threadIdx.x * NEST_SLOT_STRIDE] = 0ull;  // for the next step
     }
     __syncthreads();
     if (!s_go) return;
@@ -1023,7 +1031,7 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     if (threadIdx.x == 0) {
         unsigned long long b = wmax[0];
         for (int w = 1; w < 4; ++w) b = b > wmax[w] ? b : wmax[w];
-        atomicMax(a.slots + (size_t)(blockIdx.x & (NEST_SPEED_SLOTS - 1)) * NEST_SLOT_STRIDE, b);
+        atomicMax(a.slots + (size_t)(blockIdx.x & (kStepSlots - 1)) * NEST_SLOT_STRIDE, b);
     }
 }
 
