@@ -986,10 +986,11 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     }
     __syncthreads();
     if (a.bc != 0) {  // U1 past an outflow / reflecting wall, from the block's own cells
+        // only the ghost positions stage 2 reads; each maps to cell 0 / 1 or
+        // n - 1 / n - 2, inside the tile (nest_step1d_wall_source)
         for (int t = threadIdx.x + 2; t < T - 2; t += 256) {
-            const int g = g0 - 4 + t;
-            if (g >= 0 && g < n) continue;
-            const int ts = nest_map_index(g, n, a.bc) - (g0 - 4);
+            const int ts = nest_step1d_wall_source(t, g0, n, a.bc);
+            if (ts < 0) continue;
             const bool neg = a.bc == 2;
             C[0][t] = C[0][ts];
             C[1][t] = neg ? -C[1][ts] : C[1][ts];

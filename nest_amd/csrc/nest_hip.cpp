@@ -1059,6 +1059,11 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
         for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
         NEST_HIP_TRY(hipEventRecord(s->ev_rev_c[c], s->stream));
     }
+    // With more intervals than buffer sets, the recompute of interval
+    // nck - 1 - nsets (or an earlier one of the same set) has overwritten the
+    // kept last interval: a further reverse sweep of this rollout (a second
+    // backward of the same forward) must recompute it.
+    if (nck > nsets) s->roll_kept = false;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }

@@ -116,6 +116,20 @@ NEST_HD inline int nest_map_index(int g, int n, int bc) {
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
 
+// The fused 1D step keeps U1 in a tile of 264 positions (position t holds
+// cell g0 - 4 + t of a block whose output cells are g0 .. g0 + 255) and sets
+// U1 past an outflow / reflecting wall from the block's own boundary cells.
+// For position t: the tile position whose U1 it copies, or -1 when t needs
+// no copy (an interior cell, or a position stage 2 does not read: stage 2
+// reads g0 - 2 .. gend + 1, gend = min(n, g0 + 256)). Every source lies in
+// [3, 262) and is an interior cell.
+NEST_HD inline int nest_step1d_wall_source(int t, int g0, int n, int bc) {
+    const int g = g0 - 4 + t;
+    const int gend = n < g0 + 256 ? n : g0 + 256;
+    if (bc == 0 || (g >= 0 && g < n) || g < g0 - 2 || g >= gend + 2) return -1;
+    return nest_map_index(g, n, bc) - (g0 - 4);
+}
+
 // Variables a gathered value negates relative to its source cell: bit 1
 // (rho u) past a reflecting x wall, bit 2 (rho v) past a reflecting y wall.
 NEST_HD inline int nest_gather_flip(const nest_patch_desc& pd, const nest_prob_geom& G, int e, int g) {

@@ -11,6 +11,7 @@
 //  * the per-patch strips and the global strips each write every interior
 //    cell exactly once.
 // Exit status 0 = pass.
+#include <algorithm>
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -146,7 +147,32 @@ static void check_1d_gather() {
         }
 }
 
+// The fused 1D step's wall copies (nest_step1d_wall_source): for every size
+// and block, every ghost position stage 2 reads gets a source, and every
+// source is a tile position stage 1 computes ([2, 262)) holding an interior
+// cell, never a position the copy loop itself writes.
+static void check_step1d_wall() {
+    for (int bc = 1; bc <= 2; ++bc)
+        for (int n = 2; n <= 1100; ++n)
+            for (int g0 = 0; g0 < n; g0 += 256) {
+                const int gend = std::min(n, g0 + 256);
+                for (int t = 2; t < 262; ++t) {
+                    const int g = g0 - 4 + t;
+                    const int ts = nest_step1d_wall_source(t, g0, n, bc);
+                    const bool ghost = g < 0 || g >= n;
+                    const bool read = g >= g0 - 2 && g < gend + 2;
+                    CHECK((ts >= 0) == (ghost && read), "wall source presence n=%d g0=%d t=%d bc=%d", n, g0, t, bc);
+                    if (ts < 0) continue;
+                    const int gs = g0 - 4 + ts;
+                    CHECK(ts >= 3 && ts < 262 && gs >= 0 && gs < n, "wall source range n=%d g0=%d t=%d bc=%d ts=%d", n,
+                          g0, t, bc, ts);
+                    CHECK(gs == nest_map_index(g, n, bc), "wall source cell n=%d t=%d", n, t);
+                }
+            }
+}
+
 int main() {
+    check_step1d_wall();
     check_1d_gather();
     for (int bcx = 0; bcx < 3; ++bcx)  // periodic, outflow, reflecting
         for (int bcy = 0; bcy < 3; ++bcy) {

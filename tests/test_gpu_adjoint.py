@@ -258,3 +258,38 @@ def test_adjoint_rejects_reflecting_walls():
     with pytest.raises(NestError) as e:
         gpu_stage_vjp(sim, U, U, 1e-4)
     assert e.value.code == 5 and "reflecting" in str(e.value)
+
+
+@pytest.mark.parametrize("interval", [2, 3, 4, -1])
+def test_rollout_adjoint_twice(oracle, interval):
+    """Two reverse sweeps of one rollout (as a second backward of the same
+    forward makes) give bitwise the same cotangent. K = 11 with checkpoints
+    every 2, 3 or 4 steps is 6, 4 or 3 intervals, more than the reverse
+    sweep's two buffer sets: the first sweep's recomputes reuse the buffer
+    set the rollout kept its last interval in, so the second sweep must
+    recompute that interval. Both equal the dual-number dot-product check."""
+    nx, ny, npx, npy, K = 64, 48, 2, 2, 11
+    print_case("adjoint rollout twice", interval)
+    m = MeshSpec(nx, ny, npx, npy, bcx="periodic", bcy="periodic")
+    sim = Sim2D.mesh(nx, ny, npx, npy, bcx="periodic", bcy="periodic", mode="exact")
+    dx, dy = 1.0 / nx, 1.0 / ny
+    dt = 0.3 * min(dx, dy) / 1.5
+    U = sim.scatter(rough_state(nx, ny, seed=41, vmax=0.4))
+    sim.upload(U)
+    sim.rollout(K, dt, interval)
+    rng = np.random.default_rng(42)
+    v = rng.standard_normal((4, nx, ny))
+    w = rng.standard_normal((4, nx, ny))
+    Uc, dU = U.copy(), sim.scatter(v)
+    oracle.mesh_rk2_jvp(Uc, dU, m, K, dt, dx, dy)
+    outs = []
+    for rep in range(3):
+        Wd = dev(sim.scatter(w))
+        torch.cuda.synchronize()
+        sim.rollout_adjoint(Wd.data_ptr())
+        sim.synchronize()
+        outs.append(host(Wd))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    err = _dot_check(sim.gather(outs[0]), v, w, sim.gather(dU))
+    assert err <= TOL
+    sim.close()

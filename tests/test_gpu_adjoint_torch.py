@@ -201,3 +201,28 @@ def test_second_forward_invalidates_first_graph():
     with pytest.raises(RuntimeError):
         y1.sum().backward()
     ro.close()
+
+
+@pytest.mark.parametrize("interval", [4, 3])
+def test_two_backward_passes_retain_graph(interval):
+    """Two losses on one forward, backward with retain_graph=True: each
+    gradient equals the gradient of the same loss on a fresh forward,
+    bitwise (checkpoints every 4 or 3 steps of 12: 3 or 4 intervals, more
+    than the reverse sweep's two buffer sets)."""
+    from nest_amd.adjoint import Rollout
+    print_case("two backward passes", interval)
+    U0, _, dt = P.adjoint_setup(64)
+    ro = Rollout.mesh(64, 64, 2, 2, nsteps=12, dt=dt, mode="fast", interval=interval)
+    a = torch.from_numpy(U0).cuda().requires_grad_(True)
+    y = ro(a)
+    l1 = y[0].sum()
+    l2 = (y[3] * y[3]).sum()
+    g1 = torch.autograd.grad(l1, a, retain_graph=True)[0]
+    g2 = torch.autograd.grad(l2, a, retain_graph=True)[0]
+    g1b = torch.autograd.grad(l1, a)[0]
+    assert torch.equal(g1, g1b)
+    for loss_fn, g in ((lambda t: t[0].sum(), g1), (lambda t: (t[3] * t[3]).sum(), g2)):
+        b = torch.from_numpy(U0).cuda().requires_grad_(True)
+        gf = torch.autograd.grad(loss_fn(ro(b)), b)[0]
+        assert torch.equal(g, gf)
+    ro.close()

@@ -4,6 +4,8 @@ Exact mode must be bitwise equal to the oracle (IEEE division and sqrt, no
 FMA contraction on either side). Fast mode must agree to 1e-12 relative
 L-infinity per step (the required bound); measured deviations are ~1e-15.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -18,16 +20,21 @@ TOL_FAST = 1e-12
 # ----------------------------------------------------------------------------
 # 1D
 # ----------------------------------------------------------------------------
+LIMITERS = ["minmod", "mc", "none"]
+
+
+@pytest.mark.parametrize("lim", LIMITERS)
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 @pytest.mark.parametrize("bc", ["outflow", "periodic", "reflecting"])
-def test_1d_sod_lockstep(oracle, mode, bc):
-    """1D Sod, N=1000, CFL 0.4: same U^n and dt fed to both each step."""
+def test_1d_sod_lockstep(oracle, mode, bc, lim):
+    """1D Sod, N=1000, CFL 0.4, each limiter (euler1d.hpp:106-137): same U^n
+    and dt fed to both each step (the caller-chosen-dt stage kernels)."""
     n, cfl = 1000, 0.4
     dx = 1.0 / n
-    print_case("1d lockstep", mode, bc)
+    print_case("1d lockstep", mode, bc, lim)
     U = P.sod1d(n)
     oracle.fill_1d(U, n, bc=bc)
-    s = Solver1D(n, dx, bc=bc, mode=mode)
+    s = Solver1D(n, dx, bc=bc, mode=mode, limiter=lim)
     worst = 0.0
     for k in range(40):
         s.upload(U)
@@ -40,7 +47,7 @@ def test_1d_sod_lockstep(oracle, mode, bc):
         s.rk2_step(dt_cpu)
         got = s.download().reshape(3, -1)[:, 2:-2]
         ref = U.copy()
-        oracle.rk2_step_1d(ref, n, dt_cpu, dx, bc=bc)
+        oracle.rk2_step_1d(ref, n, dt_cpu, dx, bc=bc, lim=lim)
         want = ref.reshape(3, -1)[:, 2:-2]
         if mode == "exact":
             assert np.array_equal(got, want), f"step {k}"
@@ -50,16 +57,18 @@ def test_1d_sod_lockstep(oracle, mode, bc):
     assert worst <= TOL_FAST
 
 
+@pytest.mark.parametrize("lim", LIMITERS)
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-def test_1d_sod_run_to_t_end(oracle, mode):
-    """1D Sod, N=1000, to t = 0.2 with device-side dt: step count and final state."""
+def test_1d_sod_run_to_t_end(oracle, mode, lim):
+    """1D Sod, N=1000, to t = 0.2 with device-side dt (the fused one-launch
+    step), each limiter: step count and final state."""
     n, cfl, t_end = 1000, 0.4, 0.2
-    print_case("1d run", mode)
+    print_case("1d run", mode, lim)
     U = P.sod1d(n)
     oracle.fill_1d(U, n, bc="outflow")
     ref = U.copy()
-    k_ref, _ = oracle.run_1d(ref, n, t_end, cfl, 1.0 / n, bc="outflow")
-    s = Solver1D(n, 1.0 / n, bc="outflow", mode=mode)
+    k_ref, _ = oracle.run_1d(ref, n, t_end, cfl, 1.0 / n, bc="outflow", lim=lim)
+    s = Solver1D(n, 1.0 / n, bc="outflow", mode=mode, limiter=lim)
     s.upload(U)
     k = s.run(t_end, cfl)
     got = s.download().reshape(3, -1)[:, 2:-2]
@@ -74,24 +83,34 @@ def test_1d_sod_run_to_t_end(oracle, mode):
     assert abs(got[0].sum() - want[0].sum()) <= 1e-12 * want[0].sum()
 
 
+@pytest.mark.parametrize("lim", LIMITERS)
 @pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
-def test_1d_run_fused_step_sizes(oracle, bc):
+def test_1d_run_fused_step_sizes(oracle, bc, lim):
     """Solver1D.run takes each device-dt step in one fused launch that keeps
     U1 in LDS, alternating the state between two buffers. Bitwise equal to
     the oracle's run (exact numerics) for sizes of 2 and 3 cells (one block
     whose halo wraps / mirrors / clamps onto itself), 257 (a last block of one
     cell) and 1000, each at two end times whose step counts differ in parity
-    (the state ends in either buffer)."""
-    cases = [(2, (0.5, 0.62)), (3, (0.62, 0.7)), (257, (0.05, 0.0503)), (1000, (0.05, 0.0501))]
-    for n, t_ends in cases:
+    (the state ends in either buffer), for each limiter."""
+    # minmod on the Sod jump; MC and the unlimited scheme on a weak shock
+    # tube (on the strong jump at 2-3 cells the unlimited scheme drives the
+    # density negative, and the run ends with a non-finite dt on either side)
+    left, right = ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1)) if lim == "minmod" else ((1.0, 0.0, 1.0), (0.8, 0.0, 0.75))
+    for n, t_probe in [(2, 0.5), (3, 0.62), (257, 0.05), (1000, 0.05)]:
+        U = P.sod1d(n, left=left, right=right)
+        oracle.fill_1d(U, n, bc=bc)
+        # end times half-way into steps j and j + 1 of the oracle's dt
+        # sequence: runs of j + 1 and j + 2 steps, one of each parity
+        k0, dts = oracle.run_1d(U.copy(), n, t_probe, 0.4, 1.0 / n, bc=bc, lim=lim)
+        assert k0 >= 3 and np.isfinite(dts).all(), (n, k0)
+        T = np.concatenate([[0.0], np.cumsum(dts)])
+        j = k0 - 3
         parities = set()
-        for t_end in t_ends:
-            print_case("1d run fused", n, bc, t_end)
-            U = P.sod1d(n)
-            oracle.fill_1d(U, n, bc=bc)
+        for t_end in (T[j] + 0.5 * dts[j], T[j + 1] + 0.5 * dts[j + 1]):
+            print_case("1d run fused", n, bc, lim, t_end)
             ref = U.copy()
-            k_ref, _ = oracle.run_1d(ref, n, t_end, 0.4, 1.0 / n, bc=bc)
-            s = Solver1D(n, 1.0 / n, bc=bc, mode="exact")
+            k_ref, _ = oracle.run_1d(ref, n, t_end, 0.4, 1.0 / n, bc=bc, lim=lim)
+            s = Solver1D(n, 1.0 / n, bc=bc, mode="exact", limiter=lim)
             s.upload(U)
             k = s.run(t_end, 0.4)
             got = s.download()
@@ -538,3 +557,51 @@ def test_device_copy_kernel():
             assert dst[0].item() == -1.0
     with pytest.raises(_lib.NestError):
         _lib.check(L.nest_hip_device_copy(None, None, -1, None))
+
+
+def test_dat_files_of_gpu_states(oracle, tmp_path):
+    """The .dat outputs written from GPU-advanced states (Solver1D.write_dat,
+    Sim2D.write_dat: device -> host -> file) are byte-identical to the same
+    writers fed the oracle's state, and the writers themselves are pinned
+    byte for byte against the reference's example writers
+    (examples/sod1d/sod1d.cpp:50-91, examples/sod2d/sod2d.cpp:58-152;
+    tests/test_dat_writers.py). Exact numerics: 1D Sod N=1000 to t=0.2 with
+    device dt; 2D planar Sod 96 x 80 on 3 x 2 patches, 5 device-dt steps."""
+    from nest_amd import dat
+    print_case("dat files of GPU states")
+    gdir, cdir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    cdir.mkdir()
+    n = 1000
+    U = P.sod1d(n)
+    oracle.fill_1d(U, n, bc="outflow")
+    s = Solver1D(n, 1.0 / n, bc="outflow", mode="exact")
+    s.upload(U)
+    k = s.run(0.2, 0.4)
+    p_gpu = s.write_dat(str(gdir), 0.2, 4)
+    s.close()
+    ref = U.copy()
+    k_ref, _ = oracle.run_1d(ref, n, 0.2, 0.4, 1.0 / n, bc="outflow")
+    assert k == k_ref
+    p_cpu = dat.write_sod1d(str(cdir), ref.reshape(3, -1)[:, 2:-2], 0.2, 4)
+    assert open(p_gpu, "rb").read() == open(p_cpu, "rb").read()
+
+    nx, ny, npx, npy = 96, 80, 3, 2
+    m = MeshSpec(nx, ny, npx, npy)
+    sim = Sim2D.mesh(nx, ny, npx, npy, mode="exact")
+    U = sim.scatter(P.sod2d(nx, ny))
+    oracle.mesh_fill(U, m)
+    sim.upload(U)
+    sim.start(cfl=0.4)
+    sim.advance(5)
+    t = float(sim.status()["t"][0])
+    paths_gpu = sim.write_dat(str(gdir), t, 5)
+    sim.close()
+    ref = U.copy()
+    oracle.mesh_run(ref, m, 5, 1.0 / nx, 1.0 / ny, cfl=0.4)
+    G = m.to_global(ref)
+    paths_cpu = [dat.write_sod2d(str(cdir), G, t, 5, npx, npy), dat.write_sod2d_xslice(str(cdir), G, t, 5)]
+    assert len(paths_gpu) == 2
+    for a, b in zip(paths_gpu, paths_cpu):
+        assert os.path.basename(a) == os.path.basename(b)
+        assert open(a, "rb").read() == open(b, "rb").read(), a
