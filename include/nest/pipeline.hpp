@@ -1,57 +1,66 @@
-// nest pipelines — a tuple of stages executed over a vector of patches.
+// nest pipelines: a fixed sequence of stages run over a vector of patches.
 //
-// API-compatible with the reference's include/nest/pipeline.hpp (own
-// implementation). Stage kinds are recognised by shape, as the reference
-// does (pipeline.hpp:14-56):
-//   Compute: stage.value(patch) -> patch
-//   Reduce:  value_type, static init(), reduce(acc, const patch&) -> acc,
-//            finalize(acc, patch&)
-//   anything else: Exchange (fill every patch's halo from its plan).
-// exec::cpu and exec::omp executors are here; exec::hip (the MI355X backend)
-// is in backend_hip.hpp. The OpenMP Reduce combines per-thread accumulators
-// with min, as the reference's reduction(min:acc) does (pipeline.hpp:116).
+// Public interface of the reference's include/nest/pipeline.hpp: pipeline()
+// building a pipeline_t, execute(pipeline, patches[, exec]) and one
+// execute_stage overload per execution tag (reference pipeline.hpp:62-151).
+// The implementation is this library's own.
+//
+// The stage contract (reference pipeline.hpp:24-54) is written as C++20
+// concepts over a stage type S and the patch type P:
+//   compute_stage  s.value(P) -> P         the patch is moved in and the
+//                                          result moved back in its place
+//   reduce_stage   S::value_type, S::init(),
+//                  s.reduce(acc, const P&) -> value_type   folds one patch,
+//                  s.finalize(acc, P&)                     hands the result
+//                                                          to every patch
+//   anything else  an exchange: every patch's halo is filled from its
+//                  exchange plan (the ghost cells of a patch are written from
+//                  other patches' interiors only, so patches may be filled in
+//                  any order or in parallel).
+// A stage that is both compute- and reduce-shaped runs as compute, as in the
+// reference.
+//
+// The host executors differ only in how a loop over independent patches
+// runs and how a reduction's per-patch fold is split up; that policy is
+// host_loop<E> below, and one execute_stage serves exec::cpu and exec::omp
+// through it. Under exec::omp each thread folds its share of the patches
+// from S::init() and the partial results are combined with min, which is
+// the reference's reduction(min:acc) (pipeline.hpp:116): a Reduce stage
+// whose fold is a minimum gets the same answer from either executor.
+//
+// Stages that only exist on the device (nest::hip::*, backend_hip.hpp)
+// declare `static constexpr bool device_only = true` and are refused by the
+// host executors at compile time instead of being taken for exchanges.
 #pragma once
 #include <algorithm>
-#include <functional>
-#include <limits>
+#include <concepts>
+#include <cstddef>
 #include <tuple>
-#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "patch.hpp"
 
 namespace nest {
 
-namespace detail {
-template <typename S, typename P, typename = void>
-struct is_compute_stage : std::false_type {};
-template <typename S, typename P>
-struct is_compute_stage<S, P, std::void_t<decltype(std::declval<const S&>().value(std::declval<P>()))>>
-    : std::true_type {};
+template <class S, class P>
+concept compute_stage = requires(const S& s, P p) {
+    { s.value(std::move(p)) } -> std::convertible_to<P>;
+};
 
-template <typename S, typename P, typename = void>
-struct is_reduce_stage : std::false_type {};
-template <typename S, typename P>
-struct is_reduce_stage<
-    S, P,
-    std::void_t<typename S::value_type, decltype(S::init()),
-                decltype(std::declval<const S&>().reduce(std::declval<typename S::value_type>(),
-                                                         std::declval<const P&>())),
-                decltype(std::declval<const S&>().finalize(std::declval<typename S::value_type>(),
-                                                           std::declval<P&>()))>> : std::true_type {};
+template <class S, class P>
+concept reduce_stage = requires(const S& s, typename S::value_type acc, const P& in, P& out) {
+    { S::init() } -> std::convertible_to<typename S::value_type>;
+    { s.reduce(acc, in) } -> std::convertible_to<typename S::value_type>;
+    s.finalize(acc, out);
+};
 
-template <typename S, typename P, typename = void>
-struct is_exchange_stage : std::false_type {};
-template <typename S, typename P>
-struct is_exchange_stage<
-    S, P,
-    std::void_t<decltype(std::declval<const S&>().provides(std::declval<const P&>())),
-                decltype(std::declval<const S&>().need(std::declval<P&>(),
-                                                       std::declval<std::function<void(int, int)>>()))>>
-    : std::true_type {};
-}  // namespace detail
+template <class S>
+concept device_only_stage = requires {
+    requires S::device_only;
+};
 
-// An explicit Exchange stage (any stage that is neither Compute nor Reduce
+// An explicit exchange stage (any stage that is neither compute nor reduce
 // exchanges halos; this one names the intent).
 struct exchange_t {};
 
@@ -59,6 +68,7 @@ template <typename... Stages>
 struct pipeline_t {
     std::tuple<Stages...> stages;
     explicit pipeline_t(Stages... s) : stages(std::move(s)...) {}
+    static constexpr std::size_t size() { return sizeof...(Stages); }
 };
 
 template <typename... Stages>
@@ -66,48 +76,76 @@ pipeline_t<Stages...> pipeline(Stages... s) {
     return pipeline_t<Stages...>(std::move(s)...);
 }
 
-template <typename T, std::size_t Dim, typename Stage>
-void execute_stage(Stage& stage, std::vector<patch_t<T, Dim>>& patches, exec::cpu_t) {
-    using P = patch_t<T, Dim>;
-    if constexpr (detail::is_compute_stage<Stage, P>::value) {
-        for (auto& p : patches) p = stage.value(std::move(p));
-    } else if constexpr (detail::is_reduce_stage<Stage, P>::value) {
-        auto acc = Stage::init();
-        for (const auto& p : patches) acc = stage.reduce(acc, p);
-        for (auto& p : patches) stage.finalize(acc, p);
-    } else {
-        fill_all_halos(patches);
-    }
-}
+// How a host executor runs a loop over n independent patches (each) and a
+// fold over them (fold: acc = step(acc, i) for every i, starting from init).
+template <class E>
+struct host_loop;
 
-template <typename T, std::size_t Dim, typename Stage>
-void execute_stage(Stage& stage, std::vector<patch_t<T, Dim>>& patches, exec::omp_t) {
-    using P = patch_t<T, Dim>;
-    const long n = static_cast<long>(patches.size());
-    if constexpr (detail::is_compute_stage<Stage, P>::value) {
+template <>
+struct host_loop<exec::cpu_t> {
+    template <class F>
+    static void each(std::size_t n, F&& f) {
+        for (std::size_t i = 0; i < n; ++i) f(i);
+    }
+    template <class V, class F>
+    static V fold(std::size_t n, V init, F&& step) {
+        for (std::size_t i = 0; i < n; ++i) init = step(init, i);
+        return init;
+    }
+};
+
+template <>
+struct host_loop<exec::omp_t> {
+    template <class F>
+    static void each(std::size_t n, F&& f) {
+        const long m = static_cast<long>(n);
 #pragma omp parallel for schedule(dynamic)
-        for (long i = 0; i < n; ++i) patches[i] = stage.value(std::move(patches[i]));
-    } else if constexpr (detail::is_reduce_stage<Stage, P>::value) {
-        auto acc = Stage::init();
+        for (long i = 0; i < m; ++i) f(static_cast<std::size_t>(i));
+    }
+    // each thread folds its share from `init`; the partials meet by min
+    template <class V, class F>
+    static V fold(std::size_t n, const V& init, F&& step) {
+        const long m = static_cast<long>(n);
+        V result = init;
 #pragma omp parallel
         {
-            auto mine = Stage::init();
+            V part = init;
 #pragma omp for schedule(static) nowait
-            for (long i = 0; i < n; ++i) mine = stage.reduce(mine, patches[i]);
-#pragma omp critical(nest_reduce)
-            acc = std::min(acc, mine);
+            for (long i = 0; i < m; ++i) part = step(part, static_cast<std::size_t>(i));
+#pragma omp critical(nest_pipeline_fold)
+            result = std::min(result, part);
         }
-#pragma omp parallel for schedule(static)
-        for (long i = 0; i < n; ++i) stage.finalize(acc, patches[i]);
+        return result;
+    }
+};
+
+template <class E>
+concept host_executor = std::same_as<E, exec::cpu_t> || std::same_as<E, exec::omp_t>;
+
+template <typename T, std::size_t Dim, typename Stage, host_executor E>
+void execute_stage(Stage& stage, std::vector<patch_t<T, Dim>>& patches, E) {
+    using P = patch_t<T, Dim>;
+    using loop = host_loop<E>;
+    static_assert(!device_only_stage<Stage>, "a device-only stage (nest::hip::*) needs exec::hip");
+    const std::size_t n = patches.size();
+    if constexpr (compute_stage<Stage, P>) {
+        loop::each(n, [&](std::size_t i) { patches[i] = stage.value(std::move(patches[i])); });
+    } else if constexpr (reduce_stage<Stage, P>) {
+        using V = typename Stage::value_type;
+        const V acc = loop::fold(n, V(Stage::init()),
+                                 [&](const V& a, std::size_t i) { return V(stage.reduce(a, patches[i])); });
+        loop::each(n, [&](std::size_t i) { stage.finalize(acc, patches[i]); });
     } else {
-#pragma omp parallel for schedule(static)
-        for (long i = 0; i < n; ++i) fill_halos(patches[i], patches);
+        loop::each(n, [&](std::size_t i) { fill_halos(patches[i], patches); });
     }
 }
 
+// Run the stages in order; each sees the patches as the previous one left them.
 template <typename T, std::size_t Dim, typename... Stages, typename Exec>
 void execute(pipeline_t<Stages...>& pl, std::vector<patch_t<T, Dim>>& patches, Exec exec) {
-    std::apply([&](auto&... s) { (execute_stage<T, Dim>(s, patches, exec), ...); }, pl.stages);
+    [&]<std::size_t... I>(std::index_sequence<I...>) {
+        (execute_stage<T, Dim>(std::get<I>(pl.stages), patches, exec), ...);
+    }(std::index_sequence_for<Stages...>{});
 }
 
 template <typename T, std::size_t Dim, typename... Stages>

@@ -94,6 +94,24 @@ int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt);
 int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* steps);
 /* Use the caller's stream (hipStream_t, NULL = handle's own). */
 int nest_hip_solver1d_set_stream(nest_solver1d* s, void* stream);
+/* One SSP-RK2 stage with the given dt, the per-stage granularity of a
+   Pipeline Compute stage (the reference's 1D loop runs the stages as
+   separate passes, examples/sod1d/sod1d.cpp:219-272; benchmark_euler1d.cpp:
+   188-209): stage 1 writes U1 = S(U); stage 2 writes U = (U + S(U1)) / 2.
+   Ghost cells are read through the boundary condition. */
+int nest_hip_solver1d_stage(nest_solver1d* s, int stage, double dt);
+/* Ghost fill of U (buffer 0) or U1 (buffer 1) through the boundary
+   condition: the Exchange stage of a 1D pipeline. */
+int nest_hip_solver1d_exchange(nest_solver1d* s, int buffer);
+/* Device-dt stepping without a host round trip, by step count: start()
+   sets the CFL number, the end time (t_end = INFINITY for none) and the
+   initial wave speeds; advance() enqueues nsteps fused steps (the kernel of
+   nest_hip_solver1d_run; steps past t_end are no-ops) and leaves the
+   state in U, halo-complete. status() reads t, the last dt and the step
+   count (blocking). */
+int nest_hip_solver1d_start(nest_solver1d* s, double cfl, double t_end);
+int nest_hip_solver1d_advance(nest_solver1d* s, int nsteps);
+int nest_hip_solver1d_status(nest_solver1d* s, double* t, double* dt, int* steps, int* done);
 
 /* ---- reference-compatible entry points: exact signatures of reference
    src/backend_cuda/euler1d_solver.hpp:16-87 (periodic BC, minmod, halo 2,

@@ -952,7 +952,16 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
             a.slots_clr[(size_t)threadIdx.x * NEST_SLOT_STRIDE] = 0ull;  // for the next step
     }
     __syncthreads();
-    if (!s_go) return;
+    if (!s_go) {
+        // a step past the end (or after a failure) passes the state through,
+        // so after step k the state is in the buffer of parity k + 1
+        const int g = g0 + threadIdx.x;
+        if (g < n) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) out[k * ne + g + 2] = in[k * ne + g + 2];
+        }
+        return;
+    }
     const double dt = s_dt;
     const double dtdx = dt / a.ps->dx;
     // ---- stage 1 at t in [2, 262)

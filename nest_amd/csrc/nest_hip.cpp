@@ -1093,6 +1093,7 @@ struct nest_solver1d {
     nest_run1d_state* run_d = nullptr;      // time-stepping state of run()
     nest_run1d_state run_h{};
     hipStream_t own_stream = nullptr, stream = nullptr;
+    int k_next = 0;  // index of the next fused step of start() / advance() (buffer and slot-set parity)
     hipGraph_t graph = nullptr;  // kBatch device-dt steps, replayed by run()
     hipGraphExec_t graph_exec = nullptr;
     static constexpr int kBatch = 48;  // a multiple of 2 (buffers, time slots) and 3 (slot sets)
@@ -1228,23 +1229,31 @@ int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt) {
     return NEST_OK;
 }
 
-int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* steps) {
-    if (!s) return fail(NEST_ERR_INVALID, "null handle");
-    if (!(cfl > 0.0)) return fail(NEST_ERR_INVALID, "cfl must be > 0");
+// Time-stepping state of the fused device-dt steps: CFL number, end time,
+// step 0 next, and the speeds of the current state in slot set 2, which the
+// first step reads its dt from (sets 0 and 1 start clear).
+static int solver1d_begin(nest_solver1d* s, double cfl, double t_end) {
     nest_prob_state p = make_prob(s->dx, 1.0, 1);
-    p.t_end = t_final;
+    p.t_end = t_end;
     p.cfl = cfl;
     s->ps_h = p;
     NEST_HIP_TRY(hipMemcpyAsync(s->ps_d, &s->ps_h, sizeof(nest_prob_state), hipMemcpyHostToDevice, s->stream));
     s->run_h = nest_run1d_state{};
     NEST_HIP_TRY(hipMemcpyAsync(s->run_d, &s->run_h, sizeof(nest_run1d_state), hipMemcpyHostToDevice, s->stream));
-    // the speeds of the initial state go to slot set 2, which the first step
-    // reads its dt from; sets 0 and 1 start clear
     for (int set = 0; set < 3; ++set)
         s->L().reset_speeds(s->ps_d, s->slots_d + (size_t)set * nest_solver1d::kSet, 1, s->stream);
     stage1d_args a0 = s->args(0.0);
     a0.slots = s->slots_d + 2 * nest_solver1d::kSet;
     s->L().speeds1d(a0, s->stream);
+    s->k_next = 0;
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* steps) {
+    if (!s) return fail(NEST_ERR_INVALID, "null handle");
+    if (!(cfl > 0.0)) return fail(NEST_ERR_INVALID, "cfl must be > 0");
+    if (int rc = solver1d_begin(s, cfl, t_final)) return rc;
     if (int rc = s->ensure_graph()) return rc;
     for (;;) {
         // steps past t_end are no-ops (every launch sees the done flag)
@@ -1257,14 +1266,66 @@ int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* ste
     s->ps_h.steps = s->run_h.steps;
     s->ps_h.done = s->run_h.done;
     s->ps_h.t = s->run_h.t[s->run_h.steps & 1];
-    // steps alternate U -> U1 -> U: after an odd number of them the state is in U1
-    if (s->ps_h.steps & 1)
-        NEST_HIP_TRY(hipMemcpyAsync(s->U, s->U1, 3 * (size_t)(s->n + 4) * sizeof(double), hipMemcpyDeviceToDevice,
-                                    s->stream));
+    // launches alternate U -> U1 -> U (a step past the end passes the state
+    // through) and every graph holds an even number of them: the state is in U
     s->L().fill1d(s->U, s->n, s->bc, nullptr, s->stream);
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
     if (steps) *steps = s->ps_h.steps;
     if (s->ps_h.done == 2) return fail(NEST_ERR_NONFINITE, "non-finite dt");
+    return NEST_OK;
+}
+
+int nest_hip_solver1d_stage(nest_solver1d* s, int stage, double dt) {
+    if (!s) return fail(NEST_ERR_INVALID, "null handle");
+    if ((stage != 1 && stage != 2) || !(dt > 0.0)) return fail(NEST_ERR_INVALID, "stage must be 1|2, dt > 0");
+    s->L().stage1d(stage, s->lim, s->args(dt), s->stream);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_solver1d_exchange(nest_solver1d* s, int buffer) {
+    if (!s) return fail(NEST_ERR_INVALID, "null handle");
+    if (buffer != 0 && buffer != 1) return fail(NEST_ERR_INVALID, "buffer must be 0 (U) or 1 (U1)");
+    s->L().fill1d(buffer ? s->U1 : s->U, s->n, s->bc, nullptr, s->stream);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_solver1d_start(nest_solver1d* s, double cfl, double t_end) {
+    if (!s) return fail(NEST_ERR_INVALID, "null handle");
+    if (!(cfl > 0.0)) return fail(NEST_ERR_INVALID, "cfl must be > 0");
+    if (int rc = solver1d_begin(s, cfl, t_end)) return rc;
+    NEST_HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEST_OK;
+}
+
+int nest_hip_solver1d_advance(nest_solver1d* s, int nsteps) {
+    if (!s) return fail(NEST_ERR_INVALID, "null handle");
+    if (nsteps < 0) return fail(NEST_ERR_INVALID, "nsteps < 0");
+    const size_t bytes = 3 * (size_t)(s->n + 4) * sizeof(double);
+    // fused step k reads the buffer of parity k (U for even k, U1 for odd k);
+    // the handle's state is in U between calls
+    if (s->k_next & 1)
+        NEST_HIP_TRY(hipMemcpyAsync(s->U1, s->U, bytes, hipMemcpyDeviceToDevice, s->stream));
+    for (int i = 0; i < nsteps; ++i) s->device_step(s->k_next++);
+    if (s->k_next & 1)
+        NEST_HIP_TRY(hipMemcpyAsync(s->U, s->U1, bytes, hipMemcpyDeviceToDevice, s->stream));
+    s->L().fill1d(s->U, s->n, s->bc, nullptr, s->stream);
+    NEST_HIP_TRY(hipGetLastError());
+    return NEST_OK;
+}
+
+int nest_hip_solver1d_status(nest_solver1d* s, double* t, double* dt, int* steps, int* done) {
+    if (!s) return fail(NEST_ERR_INVALID, "null handle");
+    NEST_HIP_TRY(hipMemcpyAsync(&s->run_h, s->run_d, sizeof(nest_run1d_state), hipMemcpyDeviceToHost, s->stream));
+    NEST_HIP_TRY(hipStreamSynchronize(s->stream));
+    // step k reads the time of slot k & 1 and the last step taken wrote
+    // slot steps & 1
+    if (t) *t = s->run_h.t[s->run_h.steps & 1];
+    if (dt) *dt = s->run_h.dt;
+    if (steps) *steps = s->run_h.steps;
+    if (done) *done = s->run_h.done;
+    if (s->run_h.done == 2) return fail(NEST_ERR_NONFINITE, "non-finite dt");
     return NEST_OK;
 }
 

@@ -70,6 +70,27 @@ class Solver1D:
         from . import dat
         return dat.write_sod1d(directory, self.download().reshape(3, -1)[:, 2:-2], time, step, x_min, x_max, gamma)
 
+    def stage(self, stage: int, dt: float) -> None:
+        """One SSP-RK2 stage with the given dt (1: U1 = S(U); 2: U = (U + S(U1)) / 2)."""
+        check(self.L.nest_hip_solver1d_stage(self.h, stage, dt))
+
+    def exchange(self, buffer: int = 0) -> None:
+        """Ghost fill of U (0) or U1 (1) through the boundary condition."""
+        check(self.L.nest_hip_solver1d_exchange(self.h, buffer))
+
+    def start(self, cfl: float = 0.4, t_end: float = float("inf")) -> None:
+        check(self.L.nest_hip_solver1d_start(self.h, cfl, t_end))
+
+    def advance(self, nsteps: int) -> None:
+        """nsteps device-dt steps (no host round trip); steps past t_end are no-ops."""
+        check(self.L.nest_hip_solver1d_advance(self.h, nsteps))
+
+    def status(self):
+        t, dt = C.c_double(), C.c_double()
+        steps, done = C.c_int(), C.c_int()
+        check(self.L.nest_hip_solver1d_status(self.h, C.byref(t), C.byref(dt), C.byref(steps), C.byref(done)))
+        return {"t": t.value, "dt": dt.value, "steps": steps.value, "done": done.value}
+
     def run(self, t_final: float, cfl: float) -> int:
         k = C.c_int()
         check(self.L.nest_hip_solver1d_run(self.h, t_final, cfl, C.byref(k)))

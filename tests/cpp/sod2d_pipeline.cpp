@@ -4,7 +4,11 @@
 //   1. per-patch pipeline: cfl_reduce, exchange, rk2_stage<1>, exchange,
 //      rk2_stage<2> (one launch per patch per stage), K steps;
 //   2. fused pipeline: fused_steps{K} (one launch set, dt on the device);
-//   3. the oracle (CPU restatement of the reference numerics), K steps.
+//   3. the per-patch pipeline with the dt from a user's host Reduce stage
+//      of the reference's shape (value_type / init / reduce / finalize, fold
+//      by min), run on host copies of the patches, its result becoming the
+//      context's dt for the device stages;
+//   4. the oracle (CPU restatement of the reference numerics), K steps.
 // All three must agree bit for bit (exact numerics). Exit status 0 = pass.
 #include <cmath>
 #include <cstdio>
@@ -33,6 +37,22 @@ static std::vector<patch_t<double, 2>> make_mesh(int nx, int ny, int npx, int np
     fill_all_halos(patches);
     return patches;
 }
+
+// compute_dt_2d (euler2d.hpp:242-271) as a user's Reduce stage: per patch
+// cfl*min(min dx/(|u|+c), min dy/(|v|+c)), folded by min (bitwise the
+// whole-mesh dt: multiplication by cfl > 0 is monotone).
+struct user_dt {
+    using value_type = double;
+    double cfl, dx, dy;
+    static double init() { return 1e10; }
+    double reduce(double acc, const patch_t<double, 2>& p) const {
+        double dtx = 1e10, dty = 1e10;
+        nest_oracle::dt_2d_partial(p.state.conserved._data, static_cast<int>(shape(p.interior)[0]),
+                                   static_cast<int>(shape(p.interior)[1]), 2, dx, dy, 1.4, dtx, dty);
+        return std::min(acc, cfl * std::min(dtx, dty));
+    }
+    void finalize(double acc, patch_t<double, 2>& p) const { p.state.time = acc; }
+};
 
 static std::vector<double> flatten(const std::vector<patch_t<double, 2>>& patches) {
     std::vector<double> out;
@@ -81,7 +101,23 @@ int main() {
     execute(fused, B, exec::hip(cb));
     cb.download(B);
 
-    // 3. oracle
+    // 3. per-patch pipeline with a host Reduce stage for dt
+    auto D = make_mesh(nx, ny, npx, npy);
+    hip::context cd(D, dx, dy, 1.4, boundary_t::outflow, boundary_t::periodic);
+    cd.upload(D);
+    auto step_d = pipeline(user_dt{cfl, dx, dy}, exchange_t{}, hip::rk2_stage<1>{}, exchange_t{},
+                           hip::rk2_stage<2>{});
+    for (int k = 0; k < K; ++k) {
+        execute(step_d, D, exec::hip(cd));
+        if (cd.dt() != dts[k] || D.back().state.time != dts[k]) {
+            std::printf("FAIL: host-reduce dt of step %d: %.17g vs device %.17g\n", k, cd.dt(), dts[k]);
+            ok = false;
+        }
+    }
+    execute(ex, D, exec::hip(cd));
+    cd.download(D);
+
+    // 4. oracle
     auto C = make_mesh(nx, ny, npx, npy);
     nest_oracle::mesh2d m(nx, ny, npx, npy, 2, nest_oracle::BC_OUTFLOW, nest_oracle::BC_PERIODIC);
     std::vector<double> U = flatten(C), U1, s;
@@ -98,6 +134,10 @@ int main() {
 
     if (!same_interiors(A, C)) {
         std::printf("FAIL: per-patch pipeline differs from the oracle\n");
+        ok = false;
+    }
+    if (!same_interiors(D, C)) {
+        std::printf("FAIL: host-reduce pipeline differs from the oracle\n");
         ok = false;
     }
     if (!same_interiors(B, C)) {

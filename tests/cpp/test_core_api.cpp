@@ -196,6 +196,53 @@ TEST(pipeline_cpu_and_omp) {
     }
 }
 
+// The stage contract as concepts: classification of stage shapes.
+struct not_a_stage_value {  // value() returning something that is not a patch: not compute
+    int value(patch_t<double, 1>) const { return 0; }
+};
+struct device_tagged {
+    static constexpr bool device_only = true;
+};
+static_assert(compute_stage<scale_stage, patch_t<double, 1>>);
+static_assert(!reduce_stage<scale_stage, patch_t<double, 1>>);
+static_assert(reduce_stage<min_stage, patch_t<double, 1>>);
+static_assert(!compute_stage<min_stage, patch_t<double, 1>>);
+static_assert(!compute_stage<exchange_stage, patch_t<double, 1>> && !reduce_stage<exchange_stage, patch_t<double, 1>>);
+static_assert(!compute_stage<not_a_stage_value, patch_t<double, 1>>);
+static_assert(!compute_stage<scale_stage, patch_t<double, 2>>);  // a 1D stage is no 2D stage
+static_assert(device_only_stage<device_tagged> && !device_only_stage<exchange_stage>);
+static_assert(host_executor<exec::cpu_t> && host_executor<exec::omp_t> && !host_executor<exec::cuda_t>);
+
+struct append_stage {  // Compute that records the order stages ran in
+    int tag;
+    std::vector<int>* log;
+    patch_t<double, 1> value(patch_t<double, 1> p) const {
+        if (p.index == 0) log->push_back(tag);
+        return p;
+    }
+};
+struct count_reduce {  // Reduce whose fold is a minimum over a per-patch key
+    using value_type = long;
+    static long init() { return 1L << 40; }
+    long reduce(long acc, const patch_t<double, 1>& p) const { return std::min(acc, 1000L - 7L * p.index); }
+    void finalize(long acc, patch_t<double, 1>& p) const { p.state.time = static_cast<double>(acc); }
+};
+
+TEST(pipeline_stage_order_and_fold_equivalence) {
+    auto patches = build_periodic_mesh_1d<double>(index_space(ivec(0), uvec(640u)), 64, 1, 2);
+    std::vector<int> log;
+    auto pl = pipeline(append_stage{1, &log}, append_stage{2, &log}, exchange_t{}, append_stage{3, &log});
+    EXPECT(pl.size() == 4);
+    execute(pl, patches);
+    EXPECT((log == std::vector<int>{1, 2, 3}));
+    for (int mode = 0; mode < 2; ++mode) {
+        auto r = pipeline(count_reduce{});
+        if (mode == 0) execute(r, patches, exec::cpu);
+        else execute(r, patches, exec::omp);
+        for (auto& p : patches) EXPECT(p.state.time == 1000.0 - 7.0 * 63);
+    }
+}
+
 TEST(for_each_order_and_omp_coverage) {
     auto s = index_space(ivec(0), uvec(100u));
     std::vector<int> order;

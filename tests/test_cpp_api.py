@@ -33,21 +33,39 @@ def test_adjoint_building_blocks_match_dual_numbers():
 
 
 def test_backend_hip_header_compiles_and_links():
-    """exec::hip driver builds against include/nest/backend_hip.hpp and links
-    libnest_hip.so (no GPU needed to build)."""
-    exe = os.path.join(BIN, "sod2d_pipeline")
-    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpp"), "_bin/sod2d_pipeline"], check=True)
-    assert os.path.exists(exe)
+    """exec::hip drivers (2D mesh and 1D patch vectors) build against
+    include/nest/backend_hip.hpp and link libnest_hip.so (no GPU needed to
+    build)."""
+    for name in ("sod2d_pipeline", "sod1d_pipeline"):
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpp"), "_bin/" + name], check=True)
+        assert os.path.exists(os.path.join(BIN, name))
 
 
 @pytest.mark.gpu
 def test_exec_hip_pipeline_matches_oracle():
     """Per-patch pipeline (cfl_reduce, exchange, rk2_stage<1>, exchange,
-    rk2_stage<2>) and fused pipeline on exec::hip, bitwise vs the oracle."""
+    rk2_stage<2>), the same with a user's host Reduce stage giving dt, and
+    the fused pipeline on exec::hip, bitwise vs the oracle."""
     exe = os.path.join(BIN, "sod2d_pipeline")
     if not os.path.exists(exe):
         pytest.skip("tests/cpp/_bin/sod2d_pipeline not built")
     print("CASE: exec::hip sod2d pipeline", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_exec_hip_1d_pipeline_matches_oracle():
+    """exec::hip on 1D patch vectors (hip::context<1>): per-stage pipeline
+    with device dt, with a user's host Reduce stage giving dt, and fused
+    device-dt steps, on 4 and 3 periodic patches and one outflow patch:
+    bitwise vs the oracle, dt included, and the handed-out halos equal an
+    exchange's."""
+    exe = os.path.join(BIN, "sod1d_pipeline")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/_bin/sod1d_pipeline not built")
+    print("CASE: exec::hip sod1d pipeline", flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr

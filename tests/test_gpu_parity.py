@@ -605,3 +605,55 @@ def test_dat_files_of_gpu_states(oracle, tmp_path):
     for a, b in zip(paths_gpu, paths_cpu):
         assert os.path.basename(a) == os.path.basename(b)
         assert open(a, "rb").read() == open(b, "rb").read(), a
+
+
+@pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
+def test_1d_start_advance_and_stages(oracle, bc):
+    """The 1D solver by step count (start / advance, device dt, one fused
+    launch per step) in calls of 3, 4 and 1 steps: each call leaves the state
+    in U whatever the parity of the steps taken, and equals the oracle's
+    first 3, 7 and 8 steps bitwise; an advance that crosses t_end stops there
+    (steps past it pass the state through); the per-stage entry points
+    (stage 1, exchange of U1, stage 2) equal one oracle RK2 step."""
+    n, cfl = 400, 0.4
+    dx = 1.0 / n
+    print_case("1d start/advance/stages", bc)
+    U = P.sod1d(n)
+    oracle.fill_1d(U, n, bc=bc)
+    s = Solver1D(n, dx, bc=bc, mode="exact")
+    s.upload(U)
+    s.start(cfl)
+    done, t, ref = 0, 0.0, U.copy()
+    for calls in (3, 4, 1):
+        s.advance(calls)
+        done += calls
+        for _ in range(calls):  # the oracle's loop (sod1d.cpp:219-272)
+            dt = cfl * dx / oracle.max_speed_1d(ref, n)
+            oracle.rk2_step_1d(ref, n, dt, dx, bc=bc)
+            t += dt
+        oracle.fill_1d(ref, n, bc=bc)
+        st = s.status()
+        assert st["steps"] == done and st["dt"] == dt and st["t"] == t, (st, dt, t)
+        assert np.array_equal(s.download(), ref), (bc, done)
+    # crossing the end time: 5 requested, the run stops after the oracle's count
+    t_end = 0.004
+    ref = U.copy()
+    k_ref, _ = oracle.run_1d(ref, n, t_end, cfl, dx, bc=bc)
+    s.upload(U)
+    s.start(cfl, t_end)
+    s.advance(k_ref + 5)
+    st = s.status()
+    assert st["steps"] == k_ref and st["done"] == 1
+    assert np.array_equal(s.download(), ref)
+    # per-stage entry points
+    s.upload(U)
+    dt = s.compute_dt(cfl)
+    s.stage(1, dt)
+    s.exchange(1)
+    s.stage(2, dt)
+    s.exchange(0)
+    ref = U.copy()
+    oracle.rk2_step_1d(ref, n, dt, dx, bc=bc)
+    oracle.fill_1d(ref, n, bc=bc)
+    assert np.array_equal(s.download(), ref)
+    s.close()
