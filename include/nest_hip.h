@@ -167,15 +167,27 @@ int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int 
    and above). Both compute the same arithmetic. No counterpart in the
    reference: a backend choice, kept selectable so both paths stay tested. */
 int nest_hip_sim2d_set_stage_kernel(nest_sim2d* s, int kind);
-/* Tiling of one fused stage (1 or 2) of the device-dt / rk2_step path only;
-   0 keeps the current value. get_tiling(): rows[2], waves_per_block[2] of
-   the two fused stages (either pointer may be NULL). */
+/* How a whole RK2 step of rk2_step / advance runs: 0 automatic (one launch
+   per step, U1 kept on chip: the step kernel, whenever a state buffer is
+   below 4 GiB and the stage kernel is not forced to the register path),
+   1 two stage launches (stage 1 writes U1, stage 2 reads it). Both compute
+   the same arithmetic. No counterpart in the reference: a backend choice,
+   kept selectable so both paths stay tested. get_step_kernel(): 1 when the
+   one-launch step runs, 0 otherwise. */
+int nest_hip_sim2d_set_step_kernel(nest_sim2d* s, int kind);
+int nest_hip_sim2d_get_step_kernel(const nest_sim2d* s, int* one_launch);
+/* Tiling of one kernel of the device-dt / rk2_step path: stage 1 or 2 of
+   the two-launch step, or 3 for the one-launch step kernel; 0 keeps the
+   current value. get_tiling(): rows[2], waves_per_block[2] of the two stage
+   kernels; get_step_tiling(): those of the one-launch step (either pointer
+   may be NULL). */
 int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int waves_per_block);
-/* Per-problem rows per work item of one fused stage (array of nprob; 0 =
-   the stage default; NULL clears): lets a tuner pick the tile height per
-   problem size class inside one launch. */
+/* Per-problem rows per work item of one kernel (stage 1, 2 or 3, as above;
+   array of nprob; 0 = the kernel's default; NULL clears): lets a tuner pick
+   the tile height per problem size class inside one launch. */
 int nest_hip_sim2d_set_problem_rows(nest_sim2d* s, int stage, const int* rows);
 int nest_hip_sim2d_get_tiling(const nest_sim2d* s, int* rows, int* waves_per_block);
+int nest_hip_sim2d_get_step_tiling(const nest_sim2d* s, int* rows, int* waves_per_block);
 /* host <-> device copies of the whole concatenated buffer U (blocking) */
 int nest_hip_sim2d_upload(nest_sim2d* s, const double* host);
 int nest_hip_sim2d_download(nest_sim2d* s, double* host);
@@ -194,18 +206,21 @@ int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host);
    stage 1: U1 = S(U); stage 2: U = 0.5*(U + S(U1)). Halos of the input must
    be filled. patch < 0: all patches in one launch; else that patch only. */
 int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch);
-/* One full RK2 step with a host dt: stage 1 and stage 2 each gather their
-   halo straight from the neighbour patches / boundary through the mesh's
-   neighbour table (the same values an exchange would deliver), then one
-   halo exchange leaves U halo-complete. */
+/* One full RK2 step with a host dt: the one-launch step kernel (into U1,
+   then copied back to U) or stage 1 and stage 2 (set_step_kernel); either
+   gathers its halo straight from the neighbour patches / boundary through
+   the mesh's neighbour table (the same values an exchange would deliver),
+   then one halo exchange leaves U halo-complete. */
 int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt);
 /* Device-resident time stepping. start(): t = 0, dt policy per problem
    (fixed_dt > 0 holds dt fixed, else CFL with cfl), stop at t_end (use
    INFINITY for a fixed step count), and reduces the initial wave speeds.
-   advance(): nsteps steps of [dt finalize, stage 1, stage 2] (halos
-   gathered from neighbours, as in rk2_step) enqueued without any host
-   synchronisation, then one halo exchange of U; use_graph != 0 replays one
-   captured step graph. status(): per-problem t, dt, steps and done flags
+   advance(): nsteps steps of [dt finalize, one-launch step] (the state
+   alternating between U and U1; after an odd count it is copied back to U)
+   or of [dt finalize, stage 1, stage 2] (set_step_kernel), halos gathered
+   from neighbours as in rk2_step, enqueued without any host
+   synchronisation, then one halo exchange of U; use_graph != 0 replays a
+   captured graph of two steps (one-launch step) or one step. status(): per-problem t, dt, steps and done flags
    (arrays of nprob; any may be NULL), blocking. */
 int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_dt);
 int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph);
@@ -261,8 +276,9 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar);
 int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval, double* dt);
 /* advance() without a graph, with HIP events recorded on the handle's stream
    around each launch of the step; blocks at the end. ms[3] receives the
-   summed durations of [dt finalize, stage 1, stage 2] over the nsteps steps,
-   *total_ms the elapsed time of the whole sequence. */
+   summed durations of [dt finalize, stage 1, stage 2] over the nsteps steps
+   (one-launch step: [dt finalize, step kernel, 0]), *total_ms the elapsed
+   time of the whole sequence. */
 int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* total_ms);
 
 #ifdef __cplusplus

@@ -73,6 +73,9 @@ SIGNATURES = {
     "nest_hip_sim2d_set_tiling": (_i, [_vp, _i, _i, _i]),
     "nest_hip_sim2d_set_stage_tiling": (_i, [_vp, _i, _i, _i]),
     "nest_hip_sim2d_set_stage_kernel": (_i, [_vp, _i]),
+    "nest_hip_sim2d_set_step_kernel": (_i, [_vp, _i]),
+    "nest_hip_sim2d_get_step_kernel": (_i, [_vp, _pi]),
+    "nest_hip_sim2d_get_step_tiling": (_i, [_vp, _pi, _pi]),
     "nest_hip_sim2d_get_tiling": (_i, [_vp, _pi, _pi]),
     "nest_hip_sim2d_set_problem_rows": (_i, [_vp, _i, _pi]),
     "nest_hip_sim2d_upload": (_i, [_vp, _pd]),

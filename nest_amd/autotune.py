@@ -1,6 +1,8 @@
-"""Autotuner of the fused stage kernels' work decomposition.
+"""Autotuner of the step kernels' work decomposition.
 
-Per stage (1 and 2) it picks, by measurement on the live problem set:
+Per kernel of the step that runs (the one-launch step kernel, or stage 1 and
+stage 2 of the two-launch step) it picks, by measurement on the live problem
+set:
   * rows per work item for each problem size class (the tile height of the
     strip march: short tiles give more waves and a finer tail, long tiles
     amortise the 4-row warm-up of the X window), and
@@ -62,13 +64,16 @@ def _time(sim, steps, reps=3):
 
 
 def apply(sim, classes, cfg) -> None:
-    """Set a tuned configuration {stage: {"rows": [per class], "waves": w},
-    "persistent": 0|1}."""
+    """Set a tuned configuration {kernel: {"rows": [per class], "waves": w},
+    "persistent": 0|1}; kernel "1" / "2": the stages of the two-launch
+    step, "3": the one-launch step kernel."""
     sim.set_tiling(0, 0, cfg.get("persistent", 0))
-    for st in (1, 2):
-        c = cfg[str(st)]
-        sim.set_stage_tiling(st, 0, c["waves"])
-        sim.set_problem_rows(st, [c["rows"][k] for k in classes])
+    for st in ("1", "2", "3"):
+        if st not in cfg:
+            continue
+        c = cfg[st]
+        sim.set_stage_tiling(int(st), 0, c["waves"])
+        sim.set_problem_rows(int(st), [c["rows"][k] for k in classes])
 
 
 def _load(path):
@@ -152,18 +157,24 @@ def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = N
     """Tune `sim` (already started: start() called) in place and return the
     configuration; `sizes` = [(nx, ny)] per problem."""
     classes = size_classes(sizes)
-    key = signature(sizes, classes, arch)
+    one = sim.one_launch
+    key = ("step|" if one else "") + signature(sizes, classes, arch)
     path = _cache_path(cache)
     table = _load(path)
     if key in table:
         apply(sim, classes, table[key])
         return table[key]
     ncls = max(classes) + 1
-    cfg = {"1": {"rows": [32] * ncls, "waves": 4}, "2": {"rows": [48] * ncls, "waves": 4}}
+    if one:
+        kernels = ("3",)
+        cfg = {"3": {"rows": [32] * ncls, "waves": 4}}
+    else:
+        kernels = ("1", "2")
+        cfg = {"1": {"rows": [32] * ncls, "waves": 4}, "2": {"rows": [48] * ncls, "waves": 4}}
     apply(sim, classes, cfg)
     best = sum(_time(sim, steps))
     trials = []
-    for st in ("1", "2"):
+    for st in kernels:
         for k in range(ncls):
             if k not in classes:
                 continue

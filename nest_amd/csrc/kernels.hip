@@ -651,6 +651,403 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
 }
 
+// ---------------------------------------------------------------------------
+// One whole SSP-RK2 step per launch: U^{n+1} = 0.5 (U^n + S(S(U^n))), with
+// U1 = S(U^n) kept on chip (it never touches HBM).
+//
+// One wave owns a strip of 64 consecutive global columns (lane l holds
+// column gs + l - 4; lanes 4..59 produce output, 56 columns) and marches
+// down rows r0 .. r1 - 1 of one dim-0 partition. Iteration k runs two
+// independent chains, so the compiler can interleave them:
+//   * stage 1 of row k: the X-then-Y split stage of stage2d_dma_kernel on
+//     U^n, giving U1(k) on lanes 2..61;
+//   * stage 2 of row k - 3: the same split stage on U1, from a queue of the
+//     last three U1 rows in registers (U1(k-3) is the centre row, U1(k-1)
+//     the row entering the X window), then the RK2 combine with U^n(k-3),
+//     the wave speeds of the new state and the store.
+// U^n rows are fetched straight into a per-wave LDS ring of 8 rows (interior
+// row i in slot i & 7), two iterations before stage 1 needs them; the ring
+// still holds row k - 3 for the combine.
+//
+// Every U1 value stage 2 reads is the one the split scheme's exchange of U1
+// would deliver (the fill between the stages, sod2d.cpp:318-356): inside the
+// domain and across periodic edges stage 1 computes it directly from the
+// gathered U^n (a cell past a periodic edge sees the stencil of the cell it
+// wraps to, so it is that cell's U1 bit for bit); past an outflow or
+// reflecting edge it is copied from the boundary cells (lanes: a shuffle
+// from the lane of the mapped column; rows: the queue entry of the mapped
+// row), with the wall-normal momentum negated at a reflecting wall. Corners
+// take both mappings, as the plan's corner regions do.
+//
+// The output goes out of place (a.Uo): neighbouring strips read U^n four
+// columns and rows into this one. A problem already done (device dt) copies
+// its state through, so the two buffers stay interchangeable.
+// LDS: 8 rows x 4 variables x 64 doubles = 16 KB per wave.
+// ---------------------------------------------------------------------------
+constexpr int kStepRing = 8;
+constexpr int kWaveLdsStep = kStepRing * kRowSlot;
+// 1: the scheduler keeps stage 1 and stage 2 of an iteration apart
+#ifndef NEST_STEP_SCHED_BARRIER
+#define NEST_STEP_SCHED_BARRIER 1
+#endif
+
+template <int LIM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) void step2d_kernel(stage_args a) {
+    extern __shared__ double lds_dyn[];
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double* const ring = lds_dyn + wave * kWaveLdsStep;
+    const int hl = lane >> 1;
+    const int part = (lane & 1) * 4;
+    for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
+        const nest_item w = a.items[it];
+        const nest_patch_desc pd = a.patches[w.patch];
+        nest_prob_state* ps = a.probs + pd.prob;
+        const nest_prob_geom G = a.geoms[pd.prob];
+        const char* const Ub8 = reinterpret_cast<const char*>(a.U);
+        char* const Ob8 = reinterpret_cast<char*>(a.Uo);
+        // lane -> global column g -> the patch holding it in the home
+        // partition's rows, its local column, row length and variable stride
+        // (byte offsets: every state buffer is below 4 GiB)
+        const int gs = pd.y0 + w.c0;
+        const int g = gs + lane - 4;
+        unsigned coff;
+        int snye, svs;
+        {
+            int q, le, lc;
+            nest_gather_source(pd, G, a.itab, 2, g, q, le, lc);
+            const nest_patch_desc qd = a.patches[q];
+            snye = (qd.ny + 4) * 8;
+            svs = (qd.nx + 4) * (qd.ny + 4) * 8;
+            coff = (unsigned)((qd.off + lc) * 8);
+        }
+        const bool store = lane >= 4 && lane < 60 && g < G.NY;
+        const int r0 = w.r0, r1 = w.r1;
+        if (!(a.host_dt > 0.0) && ps->done) {
+            // finished problem: its state passes through to the output buffer
+            if (store) {
+                for (int i = r0; i < r1; ++i) {
+                    const unsigned o = coff + (unsigned)(i + 2) * (unsigned)snye;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const unsigned ov = o + (unsigned)v * (unsigned)svs;
+                        *reinterpret_cast<double*>(Ob8 + ov) = *reinterpret_cast<const double*>(Ub8 + ov);
+                    }
+                }
+            }
+            continue;
+        }
+        const double dt = a.host_dt > 0.0 ? a.host_dt : ps->dt;
+        const double dtdx = dt / ps->dx;
+        const double dtdy = dt / ps->dy;
+
+        // the dword view's source columns: strip doubles hl and 32 + hl
+        const unsigned bA = (unsigned)__shfl((int)coff, hl) + part;
+        const unsigned bB = (unsigned)__shfl((int)coff, 32 + hl) + part;
+        const int rsA = __shfl(snye, hl), rsB = __shfl(snye, 32 + hl);
+        const int vsA = __shfl(svs, hl), vsB = __shfl(svs, 32 + hl);
+
+        // interior row i of U^n into ring slot (i & 7): 8 memory operations.
+        // Rows outside the home patch resolve through the owner tables and
+        // the boundary condition (wave-uniform slow path).
+        auto issue_row = [&](int i) {
+            const int e = i + 2;
+            unsigned oA, oB;
+            int sA, sB;
+            if (e < 2 || e >= pd.nx + 2) {
+                int q, le, lc;
+                nest_gather_source(pd, G, a.itab, e, g, q, le, lc);
+                const nest_patch_desc qd = a.patches[q];
+                const int qnye = qd.ny + 4;
+                const unsigned o = (unsigned)((qd.off + (long long)le * qnye + lc) * 8);
+                const int s8 = (qd.nx + 4) * qnye * 8;
+                oA = (unsigned)__shfl((int)o, hl) + part;
+                oB = (unsigned)__shfl((int)o, 32 + hl) + part;
+                sA = __shfl(s8, hl);
+                sB = __shfl(s8, 32 + hl);
+            } else {
+                oA = bA + (unsigned)e * (unsigned)rsA;
+                oB = bB + (unsigned)e * (unsigned)rsB;
+                sA = vsA;
+                sB = vsB;
+            }
+            double* slot = ring + (i & (kStepRing - 1)) * kRowSlot;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                __builtin_amdgcn_global_load_lds(Ub8 + (oA + (unsigned)k * (unsigned)sA), (lds_void_ptr)(slot + k * 64), 4, 0,
+                                                 0);
+                __builtin_amdgcn_global_load_lds(Ub8 + (oB + (unsigned)k * (unsigned)sB), (lds_void_ptr)(slot + k * 64 + 32),
+                                                 4, 0, 0);
+            }
+        };
+        // past a reflecting wall the wall-normal momentum is negated: rho v
+        // on lanes past a y wall, rho u on rows past an x wall
+        const int sign_v = (nest_gather_flip(pd, G, 2, g) & 4) ? (int)0x80000000 : 0;
+        auto read_row = [&](int i, double* u) {
+            const double* q = ring + (i & (kStepRing - 1)) * kRowSlot + lane;
+            const int sign_u = G.bcx == 2 && (nest_gather_flip(pd, G, i + 2, g) & 2) ? (int)0x80000000 : 0;
+            u[0] = q[0];
+            u[1] = xor_sign(q[64], sign_u);
+            u[2] = xor_sign(q[128], sign_v);
+            u[3] = q[192];
+        };
+        // U^n of output row j (interior cells: nothing negated)
+        auto read_u0 = [&](int j, double* u) {
+            const double* q = ring + (j & (kStepRing - 1)) * kRowSlot + lane;
+            u[0] = q[0];
+            u[1] = q[64];
+            u[2] = q[128];
+            u[3] = q[192];
+        };
+
+        // ---- stage 1: X window over U^n, carried down the march: W1 =
+        // W(k+1), dW1 = W(k+1) - W(k), FL = left state of face k+1/2, Fm =
+        // F(k-1/2). Split stage of row k from centre Uc = U^n(k) and the new
+        // row Ub = U^n(k+2) -> V = U1(k) (euler2d.hpp:67-239).
+        prim4 W1, dW1, FL;
+        double Fm[4];
+        auto stage1 = [&](const double* Uc, const double* Ub, double* V) {
+            double ux[4];
+            {
+                prim4 W2 = prim2d(Ub[0], Ub[1], Ub[2], Ub[3], true, a.eos);
+                prim4 dW2 = sub4(W2, W1);
+                prim4 s1 = slope4<LIM>(dW1, dW2);
+                double Fp[4];
+                face_flux<true, true>(FL, face_minus(W1, s1), a.eos, Fp);
+                FL = face_plus(W1, s1);
+                ux[0] = Uc[0] - dtdx * (Fp[0] - Fm[0]);
+                ux[1] = Uc[1] - dtdx * (Fp[1] - Fm[1]);
+                ux[2] = Uc[2] - dtdx * (Fp[3] - Fm[3]);
+                ux[3] = Uc[3] - dtdx * (Fp[2] - Fm[2]);
+                W1 = W2;
+                dW1 = dW2;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Fm[k] = Fp[k];
+            }
+            prim4 Y = prim2d(ux[0], ux[1], ux[2], ux[3], false, a.eos);
+            prim4 dl = sub4(Y, shfl_up4_z(Y));
+            prim4 dr = shfl_down4_z(dl);
+            prim4 sy = slope4<LIM>(dl, dr);
+            prim4 Rf = shfl_down4_z(face_minus(Y, sy));
+            double Gf[4];
+            face_flux<true, true>(face_plus(Y, sy), Rf, a.eos, Gf);
+            double Gm[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Gm[k] = lane_up_z(Gf[k]);
+            V[0] = ux[0] - dtdy * (Gf[0] - Gm[0]);
+            V[1] = ux[1] - dtdy * (Gf[3] - Gm[3]);
+            V[2] = ux[2] - dtdy * (Gf[1] - Gm[1]);
+            V[3] = ux[3] - dtdy * (Gf[2] - Gm[2]);
+        };
+        // ---- stage 2: X window over U1 (same carry). Split stage of row j
+        // from centre Qc = U1(j) and the new row Qn = U1(j+2), RK2 combine
+        // with z = U^n(j), wave speeds, store.
+        prim4 W1b, dW1b, FLb;
+        double Fmb[4];
+        unsigned long long smx = 0, smy = 0;
+        auto stage2 = [&](int j, const double* Qc, const double* Qn) {
+            double ux[4];
+            {
+                prim4 W2 = prim2d(Qn[0], Qn[1], Qn[2], Qn[3], true, a.eos);
+                prim4 dW2 = sub4(W2, W1b);
+                prim4 s1 = slope4<LIM>(dW1b, dW2);
+                double Fp[4];
+                face_flux<true, true>(FLb, face_minus(W1b, s1), a.eos, Fp);
+                FLb = face_plus(W1b, s1);
+                ux[0] = Qc[0] - dtdx * (Fp[0] - Fmb[0]);
+                ux[1] = Qc[1] - dtdx * (Fp[1] - Fmb[1]);
+                ux[2] = Qc[2] - dtdx * (Fp[3] - Fmb[3]);
+                ux[3] = Qc[3] - dtdx * (Fp[2] - Fmb[2]);
+                W1b = W2;
+                dW1b = dW2;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Fmb[k] = Fp[k];
+            }
+            prim4 Y = prim2d(ux[0], ux[1], ux[2], ux[3], false, a.eos);
+            prim4 dl = sub4(Y, shfl_up4_z(Y));
+            prim4 dr = shfl_down4_z(dl);
+            prim4 sy = slope4<LIM>(dl, dr);
+            prim4 Rf = shfl_down4_z(face_minus(Y, sy));
+            double Gf[4];
+            face_flux<true, true>(face_plus(Y, sy), Rf, a.eos, Gf);
+            double Gm[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Gm[k] = lane_up_z(Gf[k]);
+            double o0 = ux[0] - dtdy * (Gf[0] - Gm[0]);
+            double o1 = ux[1] - dtdy * (Gf[3] - Gm[3]);
+            double o2 = ux[2] - dtdy * (Gf[1] - Gm[1]);
+            double o3 = ux[3] - dtdy * (Gf[2] - Gm[2]);
+            // SSP-RK2 combine, U^{n+1} = 0.5*(U^n + U2), U^n from the ring
+            double z[4];
+            read_u0(j, z);
+            o0 = 0.5 * (z[0] + o0);
+            o1 = 0.5 * (z[1] + o1);
+            o2 = 0.5 * (z[2] + o2);
+            o3 = 0.5 * (z[3] + o3);
+            double sx, sy2;
+            cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
+            const unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+            const unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
+            smx = (store && bx > smx) ? bx : smx;
+            smy = (store && by > smy) ? by : smy;
+            if (store) {
+                const unsigned o = coff + (unsigned)(j + 2) * (unsigned)snye;
+                *reinterpret_cast<double*>(Ob8 + o) = o0;
+                *reinterpret_cast<double*>(Ob8 + (o + (unsigned)svs)) = o1;
+                *reinterpret_cast<double*>(Ob8 + (o + 2u * (unsigned)svs)) = o2;
+                *reinterpret_cast<double*>(Ob8 + (o + 3u * (unsigned)svs)) = o3;
+            }
+        };
+
+        // U1 queue: at the start of iteration k, Q0 = U1(k-3), Q1 = U1(k-2),
+        // Q2 = U1(k-1)
+        double Q0[4], Q1[4], Q2[4];
+        // V = U1(k) on lanes past an outflow / reflecting y edge: the value
+        // of the lane of the mapped column (lanes inside the domain keep
+        // their own; lanes far past the edge receive values never read)
+        const bool yfix = G.bcy != 0 && (gs - 4 < 0 || gs + 59 >= G.NY);
+        const int ysrc = nest_step_lane_source(lane, g, G.NY, G.bcy);
+        const int yneg = G.bcy == 2 && (g < 0 || g >= G.NY) ? (int)0x80000000 : 0;
+        auto fix_lanes = [&](double* V) {
+            if (!yfix) return;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) V[v] = __shfl(V[v], ysrc);
+            V[2] = xor_sign(V[2], yneg);
+        };
+        // V = U1(k) on a row past the bottom x edge (outflow / reflecting):
+        // the queue row of the mapped row (k-1, k-2 or k-3). Rows further out
+        // are never read (stage 2 reads U1 down to global row NX + 1).
+        const int xneg = G.bcx == 2 ? (int)0x80000000 : 0;
+        auto fix_bottom = [&](int k, double* V) {
+            const int gk = pd.x0 + k;
+            if (G.bcx == 0 || gk < G.NX) return;
+            const int d = gk - nest_map_index(gk, G.NX, G.bcx);
+            if (d < 1 || d > 3) return;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) V[v] = d == 1 ? Q2[v] : (d == 2 ? Q1[v] : Q0[v]);
+            V[1] = xor_sign(V[1], xneg);
+        };
+        auto rotate = [&](const double* V) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                Q0[v] = Q1[v];
+                Q1[v] = Q2[v];
+                Q2[v] = V[v];
+            }
+        };
+        // Row fetches: U^n rows r0-4 .. r1+3 are needed (stage 1 of rows
+        // r0-2 .. r1+1). Iteration k fetches row k+4 (while that is needed)
+        // and waits for row k+2, fetched two iterations earlier; issued after
+        // it for sure: row k+3, when k+3 <= r1+3.
+        auto wait_row = [&](int k) {
+            if (k <= r1) wait_vmcnt<8>();
+            else wait_vmcnt<0>();
+        };
+        auto fetch_ahead = [&](int k) {
+            if (k + 4 <= r1 + 3) issue_row(k + 4);
+        };
+
+        // ---- prologue: rows r0-4 .. r0-1 warm the stage-1 X window up;
+        // rows r0 and r0+1 are the new rows of the first two iterations
+        issue_row(r0 - 4);
+        issue_row(r0 - 3);
+        issue_row(r0 - 2);
+        issue_row(r0 - 1);
+        wait_vmcnt<0>();
+        {
+            double A[4];
+            read_row(r0 - 4, A);
+            prim4 Wm2 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            read_row(r0 - 3, A);
+            prim4 Wm1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            read_row(r0 - 2, A);
+            prim4 W0 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            read_row(r0 - 1, A);
+            W1 = prim2d(A[0], A[1], A[2], A[3], true, a.eos);
+            wait_lgkmcnt0();
+            issue_row(r0);
+            compiler_fence();
+            issue_row(r0 + 1);
+            prim4 d0 = sub4(W0, Wm1);
+            dW1 = sub4(W1, W0);
+            prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
+            prim4 s0 = slope4<LIM>(d0, dW1);
+            face_flux<true, true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fm);
+            FL = face_plus(W0, s0);
+        }
+        // ---- stage 1 of rows r0-2 .. r0+2; at r0+1 the stage-2 X window
+        // warms up on U1 rows r0-2 .. r0+1
+        for (int k = r0 - 2; k <= r0 + 2; ++k) {
+            wait_row(k);
+            double Uc[4], Ub[4], V[4];
+            read_row(k, Uc);
+            read_row(k + 2, Ub);
+            wait_lgkmcnt0();
+            fetch_ahead(k);
+            stage1(Uc, Ub, V);
+            fix_lanes(V);
+            fix_bottom(k, V);
+            if (k == r0 + 1) {
+                // U1 rows past the top x edge (outflow / reflecting), when
+                // the chunk starts at global row 0: row -1 (Q1) <- row 0
+                // (Q2); row -2 (Q0) <- row 0 (outflow) or row 1 (V,
+                // reflecting)
+                if (G.bcx != 0 && pd.x0 + r0 == 0) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        Q0[v] = G.bcx == 2 ? V[v] : Q2[v];
+                        Q1[v] = Q2[v];
+                    }
+                    Q0[1] = xor_sign(Q0[1], xneg);
+                    Q1[1] = xor_sign(Q1[1], xneg);
+                }
+                prim4 Wm2 = prim2d(Q0[0], Q0[1], Q0[2], Q0[3], true, a.eos);
+                prim4 Wm1 = prim2d(Q1[0], Q1[1], Q1[2], Q1[3], true, a.eos);
+                prim4 W0 = prim2d(Q2[0], Q2[1], Q2[2], Q2[3], true, a.eos);
+                W1b = prim2d(V[0], V[1], V[2], V[3], true, a.eos);
+                prim4 d0 = sub4(W0, Wm1);
+                dW1b = sub4(W1b, W0);
+                prim4 s_m1 = slope4<LIM>(sub4(Wm1, Wm2), d0);
+                prim4 s0 = slope4<LIM>(d0, dW1b);
+                face_flux<true, true>(face_plus(Wm1, s_m1), face_minus(W0, s0), a.eos, Fmb);
+                FLb = face_plus(W0, s0);
+            }
+            rotate(V);
+        }
+        // ---- main march: stage 1 of row k with stage 2 of row k-3
+        for (int k = r0 + 3; k <= r1 + 1; ++k) {
+            wait_row(k);
+            double Uc[4], Ub[4], V[4];
+            read_row(k, Uc);
+            read_row(k + 2, Ub);
+            wait_lgkmcnt0();
+            fetch_ahead(k);
+            stage1(Uc, Ub, V);
+#if NEST_STEP_SCHED_BARRIER
+            __builtin_amdgcn_sched_barrier(0);
+#endif
+            stage2(k - 3, Q0, Q2);
+            fix_lanes(V);
+            fix_bottom(k, V);
+            rotate(V);
+        }
+        // ---- last output row r1-1 (its new U1 row r1+1 is Q2)
+        stage2(r1 - 1, Q0, Q2);
+        // every fetch has landed before the ring is reused (or the wave ends)
+        wait_vmcnt<0>();
+        smx = wave_max_u64(smx);
+        smy = wave_max_u64(smy);
+        if (lane == 0) {
+            unsigned long long* sl =
+                a.slots + ((size_t)pd.prob * NEST_SPEED_SLOTS + (it & (NEST_SPEED_SLOTS - 1))) * NEST_SLOT_STRIDE;
+            atomicMax(sl, smx);
+            atomicMax(sl + 1, smy);
+        }
+    }
+}
+
 // Wave speeds of the current state (first step of a run): same work items.
 __global__ __launch_bounds__(256) void speeds2d_kernel(stage_args a) {
     const int lane = threadIdx.x & 63;
@@ -1118,6 +1515,27 @@ static void stage2d_dispatch(int lim, const stage_args& a, int grid, int block, 
 void launch_stage2d(int stage, int lim, const stage_args& a, int grid, int block, hipStream_t s) {
     if (stage == 1) stage2d_dispatch<1>(lim, a, grid, block, s);
     else stage2d_dispatch<2>(lim, a, grid, block, s);
+}
+
+void launch_step2d(int lim, const stage_args& a, int grid, int block, hipStream_t s) {
+    const size_t lds = (size_t)(block / 64) * kWaveLdsStep * sizeof(double);
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((step2d_kernel<LIM_MC>), grid, block, lds, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((step2d_kernel<LIM_NONE>), grid, block, lds, s, a); break;
+        default: hipLaunchKernelGGL((step2d_kernel<LIM_MINMOD>), grid, block, lds, s, a); break;
+    }
+}
+
+int step2d_resident_blocks(int lim, int block) {
+    const size_t lds = (size_t)(block / 64) * kWaveLdsStep * sizeof(double);
+    int nb = 0;
+    hipError_t e;
+    switch (lim) {
+        case LIM_MC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_MC>, block, lds); break;
+        case LIM_NONE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_NONE>, block, lds); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_MINMOD>, block, lds); break;
+    }
+    return e == hipSuccess ? nb : 0;
 }
 
 void launch_speeds2d(const stage_args& a, int grid, int block, hipStream_t s) {

@@ -10,6 +10,8 @@
 #define NEST_DECLARE_LAUNCHERS(NS)                                                                  \
     namespace NS {                                                                                  \
     void launch_stage2d(int stage, int lim, const stage_args& a, int grid, int block, hipStream_t s); \
+    void launch_step2d(int lim, const stage_args& a, int grid, int block, hipStream_t s);           \
+    int step2d_resident_blocks(int lim, int block);                                                 \
     void launch_speeds2d(const stage_args& a, int grid, int block, hipStream_t s);                  \
     void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars, hipStream_t s); \
     void launch_finalize(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s);  \

@@ -43,6 +43,8 @@ int fail(int code, const std::string& msg) {
 // launcher table of one numerics mode
 struct launch_table {
     void (*stage2d)(int, int, const stage_args&, int, int, hipStream_t);
+    void (*step2d)(int, const stage_args&, int, int, hipStream_t);
+    int (*step2d_blocks)(int, int);
     void (*speeds2d)(const stage_args&, int, int, hipStream_t);
     void (*fill_regions)(double*, const nest_region*, int, int, hipStream_t);
     void (*finalize)(nest_prob_state*, unsigned long long*, int, hipStream_t);
@@ -57,14 +59,16 @@ struct launch_table {
                  int, hipStream_t);
 };
 
-const launch_table kExact = {nest_exact::launch_stage2d,     nest_exact::launch_speeds2d,
+const launch_table kExact = {nest_exact::launch_stage2d,     nest_exact::launch_step2d,
+                             nest_exact::step2d_resident_blocks, nest_exact::launch_speeds2d,
                              nest_exact::launch_fill_regions, nest_exact::launch_finalize,
                              nest_exact::launch_reset_speeds, nest_exact::launch_stage1d,
                              nest_exact::launch_step1d,       nest_exact::launch_speeds1d,
                              nest_exact::launch_fill1d,
                              nest_exact::launch_adj_y,        nest_exact::launch_adj_x,
                              nest_exact::launch_pack};
-const launch_table kFast = {nest_fast::launch_stage2d,     nest_fast::launch_speeds2d,
+const launch_table kFast = {nest_fast::launch_stage2d,     nest_fast::launch_step2d,
+                            nest_fast::step2d_resident_blocks, nest_fast::launch_speeds2d,
                             nest_fast::launch_fill_regions, nest_fast::launch_finalize,
                             nest_fast::launch_reset_speeds, nest_fast::launch_stage1d,
                             nest_fast::launch_step1d,       nest_fast::launch_speeds1d,
@@ -130,6 +134,7 @@ struct nest_sim2d {
     std::vector<nest_item> items_h;         // per-patch strips (own ghosts)
     std::vector<int> item_begin, item_end;  // per patch, into items_h
     std::vector<nest_item> gitems_h[2];     // global strips per stage (halo gathered from neighbours)
+    std::vector<nest_item> fitems_h;        // global strips of the one-launch step (56 output columns)
     size_t total = 0;
 
     double* U_own = nullptr;
@@ -142,6 +147,7 @@ struct nest_sim2d {
     nest_prob_state* probs_d = nullptr;
     unsigned long long* slots_d = nullptr;  // stage-2 wave-speed slots, NEST_SPEED_SLOTS per problem
     nest_item* gitems_d[2] = {nullptr, nullptr};
+    nest_item* fitems_d = nullptr;
     nest_prob_geom* geoms_d = nullptr;
     int* itab_d = nullptr;
 
@@ -153,11 +159,15 @@ struct nest_sim2d {
     int rows = 64;              // per-patch strips
     int wpb = 4;
     int grows[2] = {32, 48};    // global strips, rows per item, stage 1 / stage 2
-    std::vector<int> prob_rows[2];  // optional per-problem override of grows (0 = default)
     int gwpb[2] = {4, 4};       // waves per block, stage 1 / stage 2
+    int frows = 32, fwpb = 4;   // the one-launch step kernel
+    std::vector<int> prob_rows[3];  // optional per-problem override of grows / frows (0 = default)
     int persistent = 0;
-    int resident_waves = 0;
+    int resident_waves = 0;     // resident waves of the stage kernels (persistent launches)
+    int fresident = 0;          // resident waves of the step kernel
+    int ncu = 0;
     int stage_kernel = 0;  // 0 automatic, 1 register path, 2 LDS-ring path
+    int step_kernel = 0;   // 0 automatic (one launch per step where it applies), 1 two stage launches
 
     // ---- discrete adjoint of fixed-dt rollouts
     std::vector<nest_item> aitems_h[2];  // adjoint strips: Y kernel (58 owned columns), X kernel (64)
@@ -193,7 +203,8 @@ struct nest_sim2d {
     void release() {
         drop_graph();
         for (void* p : {(void*)U_own, (void*)U1_own, (void*)patches_d, (void*)regions_d, (void*)items_d,
-                        (void*)probs_d, (void*)slots_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)geoms_d, (void*)itab_d,
+                        (void*)probs_d, (void*)slots_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)fitems_d,
+                        (void*)geoms_d, (void*)itab_d,
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
                         (void*)goff_d})
             if (p) (void)hipFree(p);
@@ -220,6 +231,7 @@ struct nest_sim2d {
         U_own = U1_own = U = U1 = nullptr;
         patches_d = nullptr;
         gitems_d[0] = gitems_d[1] = nullptr;
+        fitems_d = nullptr;
         geoms_d = nullptr;
         itab_d = nullptr;
         regions_d = nullptr;
@@ -253,10 +265,20 @@ struct nest_sim2d {
         a.kernel = stage_kernel;
         return a;
     }
-    int grid_for(int nitems, int waves) const {
+    // blocks for nitems work items at `waves` waves per block; persistent:
+    // at most the resident waves (`resident`, default those of the stage
+    // kernels), each wave looping over the work list
+    int grid_for(int nitems, int waves, int resident = -1) const {
         int blocks = (nitems + waves - 1) / waves;
-        if (persistent && resident_waves > 0) blocks = std::min(blocks, std::max(1, resident_waves / waves));
+        const int res = resident < 0 ? resident_waves : resident;
+        if (persistent && res > 0) blocks = std::min(blocks, std::max(1, res / waves));
         return std::max(blocks, 1);
+    }
+    // a whole RK2 step in one launch (step2d_kernel): chosen unless the
+    // two-launch step or the register-path stage kernel is asked for; it
+    // addresses the state buffers with 32-bit byte offsets
+    bool one_launch() const {
+        return step_kernel == 0 && stage_kernel != 1 && (unsigned long long)total * sizeof(double) < (1ULL << 32);
     }
 
     // Work list: per patch, row chunks of `rows`, strips of 60 output columns.
@@ -276,13 +298,18 @@ struct nest_sim2d {
                 nest_plan::add_gather_items(gitems_h[st], tb, q, r);
             }
         }
+        fitems_h.clear();
+        for (int q = 0; q < (int)tb.geoms.size(); ++q) {
+            const int r = (int)prob_rows[2].size() == nprob && prob_rows[2][q] > 0 ? prob_rows[2][q] : frows;
+            nest_plan::add_gather_items(fitems_h, tb, q, r, 56);
+        }
         for (int k = 0; k < 2; ++k) {
             aitems_h[k].clear();
             for (int q = 0; q < (int)tb.geoms.size(); ++q)
                 nest_plan::add_gather_items(aitems_h[k], tb, q, arows[k], k == 0 ? 58 : 64);
         }
         if (int rc = validate_tables()) return rc;
-        for (nest_item** d : {&items_d, &gitems_d[0], &gitems_d[1], &aitems_d[0], &aitems_d[1]}) {
+        for (nest_item** d : {&items_d, &gitems_d[0], &gitems_d[1], &fitems_d, &aitems_d[0], &aitems_d[1]}) {
             if (*d) (void)hipFree(*d);
             *d = nullptr;
         }
@@ -296,6 +323,9 @@ struct nest_sim2d {
             NEST_HIP_TRY(hipMemcpy(aitems_d[st], aitems_h[st].data(), aitems_h[st].size() * sizeof(nest_item),
                                    hipMemcpyHostToDevice));
         }
+        if (int rc = dev_alloc(&fitems_d, fitems_h.size())) return rc;
+        NEST_HIP_TRY(hipMemcpy(fitems_d, fitems_h.data(), fitems_h.size() * sizeof(nest_item), hipMemcpyHostToDevice));
+        fresident = L().step2d_blocks(lim, 64 * fwpb) * fwpb * ncu;
         drop_graph();
         return NEST_OK;
     }
@@ -304,7 +334,8 @@ struct nest_sim2d {
     // on the host before anything is uploaded or launched.
     int validate_tables() const {
         if (!nest_plan::tables_in_bounds(tb, items_h) || !nest_plan::tables_in_bounds(tb, gitems_h[0]) ||
-            !nest_plan::tables_in_bounds(tb, gitems_h[1]) || !nest_plan::tables_in_bounds(tb, aitems_h[0]) ||
+            !nest_plan::tables_in_bounds(tb, gitems_h[1]) || !nest_plan::tables_in_bounds(tb, fitems_h) ||
+            !nest_plan::tables_in_bounds(tb, aitems_h[0]) ||
             !nest_plan::tables_in_bounds(tb, aitems_h[1]) ||
             (long long)total != tb.total)
             return fail(NEST_ERR_INVALID, "internal: a patch / halo region / work item addresses memory "
@@ -351,6 +382,7 @@ struct nest_sim2d {
         int dev = 0, cus = 0;
         NEST_HIP_TRY(hipGetDevice(&dev));
         NEST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ncu = cus;
         resident_waves = cus * 4 * 4;  // 4 SIMDs per CU, 4 waves/SIMD (the LDS-ring stage kernels)
         return build_items();
     }
@@ -372,13 +404,37 @@ struct nest_sim2d {
         if (nitems <= 0) return;
         L().speeds2d(args(0.0, item0, nitems), grid_for(nitems, wpb), 64 * wpb, stream);
     }
-    // One fused step: device dt, then the two stages gathering their halos
+    // one-launch step from src into dst (all global strips)
+    void fstep(double host_dt, const double* src, double* dst) {
+        const int n = (int)fitems_h.size();
+        stage_args a = args(host_dt, 0, n, 0);
+        a.items = fitems_d;
+        a.gather = 1;
+        a.U = const_cast<double*>(src);
+        a.U1 = nullptr;
+        a.Uo = dst;
+        L().step2d(lim, a, grid_for(n, fwpb, fresident), 64 * fwpb, stream);
+    }
+    // Step k of a device-dt sequence: dt finalize, then the one-launch step
+    // from U into U1 (even k) or from U1 into U (odd k), or the two stages
+    // (stage 1 U -> U1, stage 2 U1, U -> U). Every kernel gathers its halo
     // from the neighbours (no exchange launches). Ghost cells of U are left
     // stale; callers refresh them with fill(U) when they hand U out.
-    void device_step() {
+    void device_step(int k) {
         L().finalize(probs_d, slots_d, nprob, stream);
-        gstage(1, 0.0);
-        gstage(2, 0.0);
+        if (one_launch()) {
+            fstep(0.0, (k & 1) ? U1 : U, (k & 1) ? U : U1);
+        } else {
+            gstage(1, 0.0);
+            gstage(2, 0.0);
+        }
+    }
+    // after a device-dt sequence of nsteps one-launch steps the state is in
+    // U1 when nsteps is odd: bring it back to U
+    int settle(int nsteps) {
+        if (one_launch() && (nsteps & 1))
+            NEST_HIP_TRY(hipMemcpyAsync(U, U1, total * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        return NEST_OK;
     }
 
     // ---- adjoint and fixed-dt rollouts ----
@@ -633,9 +689,9 @@ int nest_hip_sim2d_set_tiling(nest_sim2d* s, int rows, int waves_per_block, int 
     // the stage kernels are compiled with __launch_bounds__(256): at most 4 waves per block
     if (rows < 0 || waves_per_block < 0 || waves_per_block > 4)
         return fail(NEST_ERR_INVALID, "rows >= 1, 1 <= waves_per_block <= 4");
-    if (waves_per_block) s->wpb = s->gwpb[0] = s->gwpb[1] = waves_per_block;
+    if (waves_per_block) s->wpb = s->gwpb[0] = s->gwpb[1] = s->fwpb = waves_per_block;
     s->persistent = persistent;
-    if (rows) s->rows = s->grows[0] = s->grows[1] = rows;
+    if (rows) s->rows = s->grows[0] = s->grows[1] = s->frows = rows;
     return s->build_items();
 }
 
@@ -649,18 +705,44 @@ int nest_hip_sim2d_set_stage_kernel(nest_sim2d* s, int kind) {
     return NEST_OK;
 }
 
+int nest_hip_sim2d_set_step_kernel(nest_sim2d* s, int kind) {
+    if (int rc = valid_sim(s)) return rc;
+    if (kind != 0 && kind != 1) return fail(NEST_ERR_INVALID, "kind: 0 automatic (one launch per step), 1 two stage launches");
+    s->step_kernel = kind;
+    s->drop_graph();
+    return NEST_OK;
+}
+
+int nest_hip_sim2d_get_step_kernel(const nest_sim2d* s, int* one_launch) {
+    if (int rc = valid_sim(s)) return rc;
+    if (one_launch) *one_launch = s->one_launch() ? 1 : 0;
+    return NEST_OK;
+}
+
 int nest_hip_sim2d_set_stage_tiling(nest_sim2d* s, int stage, int rows, int waves_per_block) {
     if (int rc = valid_sim(s)) return rc;
-    if ((stage != 1 && stage != 2) || rows < 0 || waves_per_block < 0 || waves_per_block > 4)
-        return fail(NEST_ERR_INVALID, "stage 1|2, rows >= 1, 1 <= waves_per_block <= 4");
-    if (waves_per_block) s->gwpb[stage - 1] = waves_per_block;
-    if (rows) s->grows[stage - 1] = rows;
+    if (stage < 1 || stage > 3 || rows < 0 || waves_per_block < 0 || waves_per_block > 4)
+        return fail(NEST_ERR_INVALID, "stage 1|2 (or 3: the one-launch step), rows >= 1, 1 <= waves_per_block <= 4");
+    if (stage == 3) {
+        if (waves_per_block) s->fwpb = waves_per_block;
+        if (rows) s->frows = rows;
+    } else {
+        if (waves_per_block) s->gwpb[stage - 1] = waves_per_block;
+        if (rows) s->grows[stage - 1] = rows;
+    }
     return s->build_items();
+}
+
+int nest_hip_sim2d_get_step_tiling(const nest_sim2d* s, int* rows, int* waves_per_block) {
+    if (int rc = valid_sim(s)) return rc;
+    if (rows) *rows = s->frows;
+    if (waves_per_block) *waves_per_block = s->fwpb;
+    return NEST_OK;
 }
 
 int nest_hip_sim2d_set_problem_rows(nest_sim2d* s, int stage, const int* rows) {
     if (int rc = valid_sim(s)) return rc;
-    if (stage != 1 && stage != 2) return fail(NEST_ERR_INVALID, "stage must be 1 or 2");
+    if (stage < 1 || stage > 3) return fail(NEST_ERR_INVALID, "stage must be 1, 2 or 3");
     if (!rows) {
         s->prob_rows[stage - 1].clear();
     } else {
@@ -757,8 +839,13 @@ int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch) {
 int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt) {
     if (int rc = valid_sim(s)) return rc;
     if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
-    s->gstage(1, dt);
-    s->gstage(2, dt);
+    if (s->one_launch()) {
+        s->fstep(dt, s->U, s->U1);
+        NEST_HIP_TRY(hipMemcpyAsync(s->U, s->U1, s->total * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    } else {
+        s->gstage(1, dt);
+        s->gstage(2, dt);
+    }
     s->fill(s->U);
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
@@ -791,17 +878,22 @@ int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph) {
     if (int rc = valid_sim(s)) return rc;
     if (nsteps < 0) return fail(NEST_ERR_INVALID, "nsteps < 0");
     if (!use_graph) {
-        for (int k = 0; k < nsteps; ++k) s->device_step();
+        for (int k = 0; k < nsteps; ++k) s->device_step(k);
     } else {
+        // the graph holds one step, or two one-launch steps (U -> U1 -> U);
+        // an odd last step runs outside it
+        const int per = s->one_launch() ? 2 : 1;
         if (!s->graph_exec) {
             NEST_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            s->device_step();
+            for (int k = 0; k < per; ++k) s->device_step(k);
             hipError_t e = hipStreamEndCapture(s->stream, &s->graph);
             if (e != hipSuccess) return fail(NEST_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
             NEST_HIP_TRY(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
         }
-        for (int k = 0; k < nsteps; ++k) NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+        for (int r = 0; r < nsteps / per; ++r) NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+        for (int k = nsteps / per * per; k < nsteps; ++k) s->device_step(k);
     }
+    if (int rc = s->settle(nsteps)) return rc;
     if (nsteps > 0) s->fill(s->U);  // the fused steps leave U's ghost cells stale
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
@@ -859,16 +951,23 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) rc = fail(NEST_ERR_HIP, "hipEventCreate failed");
     if (!rc) {
-        for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(), with events between
+        const bool one = s->one_launch();
+        for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(k), with events between
             hipEvent_t* e = &ev[4 * (size_t)k];
             (void)hipEventRecord(e[0], s->stream);
             s->L().finalize(s->probs_d, s->slots_d, s->nprob, s->stream);
             (void)hipEventRecord(e[1], s->stream);
-            s->gstage(1, 0.0);
-            (void)hipEventRecord(e[2], s->stream);
-            s->gstage(2, 0.0);
+            if (one) {
+                s->fstep(0.0, (k & 1) ? s->U1 : s->U, (k & 1) ? s->U : s->U1);
+                (void)hipEventRecord(e[2], s->stream);
+            } else {
+                s->gstage(1, 0.0);
+                (void)hipEventRecord(e[2], s->stream);
+                s->gstage(2, 0.0);
+            }
             (void)hipEventRecord(e[3], s->stream);
         }
+        if (int r2 = s->settle(nsteps)) rc = r2;
         hipError_t e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) rc = fail(NEST_ERR_HIP, std::string("advance_timed: ") + hipGetErrorString(e));
     }

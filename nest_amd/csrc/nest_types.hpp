@@ -130,6 +130,17 @@ NEST_HD inline int nest_step1d_wall_source(int t, int g0, int n, int bc) {
     return nest_map_index(g, n, bc) - (g0 - 4);
 }
 
+// The one-launch 2D step keeps U1 of a strip's 64 lanes (lane l holds global
+// column gs + l - 4) in registers; past an outflow / reflecting y edge a
+// lane takes U1 from the lane of its mapped column. The lane a lane of
+// global column g reads (itself inside the domain; clamped into the wave
+// for lanes far past the edge, whose values are never read).
+NEST_HD inline int nest_step_lane_source(int lane, int g, int ny, int bcy) {
+    if (bcy == 0 || (g >= 0 && g < ny)) return lane;
+    const int s = lane + (nest_map_index(g, ny, bcy) - g);
+    return s < 0 ? 0 : (s > 63 ? 63 : s);
+}
+
 // Variables a gathered value negates relative to its source cell: bit 1
 // (rho u) past a reflecting x wall, bit 2 (rho v) past a reflecting y wall.
 NEST_HD inline int nest_gather_flip(const nest_patch_desc& pd, const nest_prob_geom& G, int e, int g) {

@@ -223,11 +223,33 @@ class Sim2D:
         """Which 2D stage kernel runs: "auto", "register" or "lds_ring"."""
         check(self.L.nest_hip_sim2d_set_stage_kernel(self.h, {"auto": 0, "register": 1, "lds_ring": 2}[kind]))
 
+    def set_step_kernel(self, kind: str = "auto") -> None:
+        """How a whole RK2 step runs: "auto" (one launch per step, U1 kept
+        on chip, where it applies) or "two_stage" (stage 1 writes U1, stage
+        2 reads it)."""
+        check(self.L.nest_hip_sim2d_set_step_kernel(self.h, {"auto": 0, "two_stage": 1}[kind]))
+
+    @property
+    def one_launch(self) -> bool:
+        """True when rk2_step / advance run the one-launch step kernel."""
+        v = C.c_int()
+        check(self.L.nest_hip_sim2d_get_step_kernel(self.h, C.byref(v)))
+        return bool(v.value)
+
     def set_stage_tiling(self, stage: int, rows: int = 0, waves_per_block: int = 0) -> None:
+        """Rows per work item and waves per block of stage 1 or 2 of the
+        two-launch step, or of the one-launch step kernel (stage 3)."""
         check(self.L.nest_hip_sim2d_set_stage_tiling(self.h, stage, rows, waves_per_block))
 
+    def get_step_tiling(self):
+        """(rows, waves per block) of the one-launch step kernel."""
+        r, w = C.c_int(), C.c_int()
+        check(self.L.nest_hip_sim2d_get_step_tiling(self.h, C.byref(r), C.byref(w)))
+        return r.value, w.value
+
     def set_problem_rows(self, stage: int, rows) -> None:
-        """Rows per work item for each problem of a batch (None clears)."""
+        """Rows per work item for each problem of a batch (None clears), of
+        stage 1, 2 or the one-launch step (3)."""
         if rows is None:
             check(self.L.nest_hip_sim2d_set_problem_rows(self.h, stage, None))
             return

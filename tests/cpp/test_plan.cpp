@@ -33,17 +33,19 @@ static int failures = 0;
 
 using nest_plan::tables;
 
-// lanes 2..61 of every item: which interior cells do they write?
-static void check_coverage(const tables& t, const std::vector<nest_item>& I, bool gather, const char* what) {
+// output lanes of every item (2..61; the one-launch step's strips of 56
+// columns: 4..59): which interior cells do they write?
+static void check_coverage(const tables& t, const std::vector<nest_item>& I, bool gather, const char* what,
+                           int lane0 = 2) {
     std::vector<int> cover(t.total / 4, 0);
     for (const auto& it : I) {
         const auto& pd = t.patches[it.patch];
         const auto& G = t.geoms[pd.prob];
-        for (int lane = 2; lane < 62; ++lane) {
+        for (int lane = lane0; lane < 64 - lane0; ++lane) {
             long long off;
             int lc, nye;
             if (gather) {
-                const int g = pd.y0 + it.c0 + lane - 2;
+                const int g = pd.y0 + it.c0 + lane - lane0;
                 if (g >= G.NY) continue;
                 int q, le;
                 nest_gather_source(pd, G, t.itab.data(), 2, g, q, le, lc);
@@ -65,6 +67,45 @@ static void check_coverage(const tables& t, const std::vector<nest_item>& I, boo
                 CHECK(cover[d.off / 4 + (long long)(i + 2) * (d.ny + 4) + j + 2] == 1, "%s coverage", what);
 }
 
+// The one-launch step's strips (56 output columns): every address its
+// gather forms (lanes 0..63, U^n rows r0-4 .. r1+3) is an interior cell of
+// a patch of the problem; lanes resolve their column within the home
+// partition's rows; every lane stage 2 reads (2..61) that takes U1 from
+// another lane (nest_step_lane_source) takes it from a lane stage 1
+// computes (2..61) holding the mapped column.
+static void check_step_items(const tables& t, const std::vector<nest_item>& FI, const char* what) {
+    const int* T = t.itab.data();
+    for (const auto& it : FI) {
+        const auto& pd = t.patches[it.patch];
+        const auto& G = t.geoms[pd.prob];
+        for (int lane = 0; lane < 64; ++lane) {
+            const int g = pd.y0 + it.c0 + lane - 4;
+            int q, le, lc;
+            nest_gather_source(pd, G, T, 2, g, q, le, lc);
+            CHECK(q >= G.patch_base && q < G.patch_base + G.npx * G.npy && t.patches[q].nx == pd.nx &&
+                      le == 2 && lc >= 2 && lc < t.patches[q].ny + 2,
+                  "%s: lane column source", what);
+            for (int i = it.r0 - 4; i <= it.r1 + 3; ++i) {
+                nest_gather_source(pd, G, T, i + 2, g, q, le, lc);
+                const auto& qd = t.patches[q];
+                CHECK(q >= G.patch_base && q < G.patch_base + G.npx * G.npy && le >= 2 && le < qd.nx + 2 && lc >= 2 &&
+                          lc < qd.ny + 2 && qd.off + 3LL * (qd.nx + 4) * (qd.ny + 4) + (long long)le * (qd.ny + 4) + lc < t.total,
+                      "%s: row source", what);
+            }
+            if (lane >= 2 && lane < 62) {
+                const int s = nest_step_lane_source(lane, g, G.NY, G.bcy);
+                const int gs = pd.y0 + it.c0 + s - 4;
+                const bool inside = g >= 0 && g < G.NY;
+                // only lanes stage 2 reads on behalf of a stored output (g <= NY + 1) matter
+                if (!inside && G.bcy != 0 && g >= -2 && g <= G.NY + 1)
+                    CHECK(s >= 2 && s < 62 && gs == nest_map_index(g, G.NY, G.bcy), "%s: lane source", what);
+                if (inside || G.bcy == 0) CHECK(s == lane, "%s: lane source identity", what);
+            }
+        }
+    }
+    check_coverage(t, FI, true, what, 4);
+}
+
 static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int rows) {
     tables t;
     nest_plan::build_mesh(t, nx, ny, npx, npy, bcx, bcy);
@@ -76,6 +117,12 @@ static void check_mesh(int nx, int ny, int npx, int npy, int bcx, int bcy, int r
     CHECK(t.regions.size() == 8 * t.patches.size(), "region count");
     check_coverage(t, I, false, "patch strips");
     check_coverage(t, GI, true, "global strips");
+    for (int r : {rows, 1, 4, 7}) {
+        std::vector<nest_item> FI;
+        nest_plan::add_gather_items(FI, t, 0, r, 56);
+        CHECK(nest_plan::tables_in_bounds(t, FI), "step strip bounds");
+        check_step_items(t, FI, "step strips");
+    }
     // region execution == oracle fill
     nest_oracle::mesh2d m(nx, ny, npx, npy, 2, bcx, bcy);
     CHECK((long long)m.total == t.total, "total %lld vs %zu", t.total, m.total);
@@ -197,6 +244,10 @@ int main() {
         for (int k = 0; k < 8; ++k) CHECK(t.patches[k].prob == k && t.geoms[k].patch_base == k, "batch indices");
         check_coverage(t, I, false, "batch patch strips");
         check_coverage(t, GI, true, "batch global strips");
+        std::vector<nest_item> FI;
+        for (int k = 0; k < 8; ++k) nest_plan::add_gather_items(FI, t, k, 12, 56);
+        CHECK(nest_plan::tables_in_bounds(t, FI), "batch step strip bounds");
+        check_step_items(t, FI, "batch step strips");
         std::vector<double> a(t.total), b;
         std::mt19937_64 g(7);
         std::uniform_real_distribution<double> U(-1.0, 1.0);

@@ -28,3 +28,26 @@ def rel_linf_vars(a, b):
         num = np.max(np.abs(a[k] - b[k]))
         out = max(out, num / den if den > 0 else num)
     return out
+
+
+# How a 2D RK2 step runs on the GPU: the one-launch step kernel (U1 kept on
+# chip; the default), the two stage kernels with rows fetched into LDS, or
+# the two register-path stage kernels (used for state buffers of 4 GiB and
+# more).
+STEP_PATHS = ["one_launch", "two_stage", "register"]
+
+
+def configure(sim, path):
+    """Select one of STEP_PATHS on a Sim2D handle."""
+    if path == "one_launch":
+        sim.set_stage_kernel("auto")
+        sim.set_step_kernel("auto")
+        assert sim.one_launch
+    elif path == "two_stage":
+        sim.set_stage_kernel("lds_ring")
+        sim.set_step_kernel("two_stage")
+        assert not sim.one_launch
+    else:
+        sim.set_stage_kernel("register")
+        sim.set_step_kernel("two_stage")
+        assert not sim.one_launch

@@ -11,7 +11,7 @@ import pytest
 
 from nest_amd import Sim2D, Solver1D, problems as P
 from oracle.oracle import MeshSpec
-from tests.gpu_util import print_case, rel_linf_vars
+from tests.gpu_util import STEP_PATHS, configure, print_case, rel_linf_vars
 
 pytestmark = pytest.mark.gpu
 TOL_FAST = 1e-12
@@ -216,9 +216,8 @@ def _state(kind, nx, ny):
     return np.ascontiguousarray(G)
 
 
-# "auto" runs the LDS-ring stage kernel at these sizes; "register" forces the
-# register-path kernel (used for state buffers of 4 GiB and more)
-STAGE_KERNELS = ["auto", "register"]
+# every way a step can run (tests/gpu_util.py: STEP_PATHS)
+STAGE_KERNELS = STEP_PATHS
 
 
 @pytest.mark.parametrize("case", CASES_2D)
@@ -229,7 +228,7 @@ def test_2d_mesh_lockstep(oracle, case, mode, kernel):
     print_case("2d lockstep", case, mode, kernel)
     dx, dy = 1.0 / nx, 1.0 / ny
     sim = Sim2D.mesh(nx, ny, npx, npy, dx, dy, bcx=bcx, bcy=bcy, mode=mode)
-    sim.set_stage_kernel(kernel)
+    configure(sim, kernel)
     m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
     U = sim.scatter(_state(kind, nx, ny))
     oracle.mesh_fill(U, m)
@@ -271,7 +270,7 @@ def test_2d_minimum_size_patches(oracle, kernel):
                 print_case("2d minimum patches", (nx, ny, npx, npy), bcx, bcy, kernel)
                 dx, dy = 1.0 / nx, 1.0 / ny
                 sim = Sim2D.mesh(nx, ny, npx, npy, dx, dy, bcx=bcx, bcy=bcy, mode="exact")
-                sim.set_stage_kernel(kernel)
+                configure(sim, kernel)
                 m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
                 G = P.quadrants(max(nx, ny), nx * 7 + ny)[:, :nx, :ny].copy()
                 U = sim.scatter(G)
@@ -340,7 +339,7 @@ def test_2d_device_dt_run(oracle, graph, kernel):
     nx, ny, npx, npy = 64, 48, 2, 2
     print_case("2d device dt", graph, kernel)
     sim = Sim2D.mesh(nx, ny, npx, npy)
-    sim.set_stage_kernel(kernel)
+    configure(sim, kernel)
     m = MeshSpec(nx, ny, npx, npy)
     U = sim.scatter(P.sod2d(nx, ny))
     oracle.mesh_fill(U, m)
@@ -362,7 +361,7 @@ def test_2d_limiters(oracle, kernel):
         for mode in ["exact", "fast"]:
             print_case("limiter", lim, mode, kernel)
             sim = Sim2D.mesh(nx, ny, 2, 2, limiter=lim, mode=mode, bcx="periodic", bcy="periodic")
-            sim.set_stage_kernel(kernel)
+            configure(sim, kernel)
             m = MeshSpec(nx, ny, 2, 2, bcx="periodic", bcy="periodic")
             U = sim.scatter(P.smooth_periodic(48)[:, :nx, :ny].copy())
             oracle.mesh_fill(U, m)

@@ -1,9 +1,10 @@
-"""GPU parity of every work decomposition the stage kernels can be given.
+"""GPU parity of every work decomposition the step kernels can be given.
 
-The autotuner (nest_amd.autotune) picks, per stage, the rows per work item
-(autotune.ROWS), the waves per block (autotune.WAVES) and whether the launch
-is persistent (grid capped at the resident waves, each wave looping over the
-work list). Each of those choices changes where strips start and end, how
+The autotuner (nest_amd.autotune) picks, per kernel (the one-launch step
+kernel, or stage 1 and stage 2 of the two-launch step), the rows per work
+item (autotune.ROWS), the waves per block (autotune.WAVES) and whether the
+launch is persistent (grid capped at the resident waves, each wave looping
+over the work list). Each of those choices changes where strips start and end, how
 many rows a strip's X window warms up over, and which wave runs which item;
 none may change a single bit of the result. Every choice runs here through
 the device-dt path (start / advance, the path bench.py times) on an uneven
@@ -21,7 +22,7 @@ import pytest
 
 from nest_amd import Sim2D, autotune, problems as P
 from oracle.oracle import MeshSpec
-from tests.gpu_util import print_case, rel_linf_vars
+from tests.gpu_util import STEP_PATHS, configure, print_case, rel_linf_vars
 
 pytestmark = pytest.mark.gpu
 TOL_FAST = 1e-12
@@ -91,20 +92,21 @@ def _same(got, want, mode, what):
 
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-@pytest.mark.parametrize("kernel", ["auto", "register"])
+@pytest.mark.parametrize("kernel", STEP_PATHS)
 @pytest.mark.parametrize("persistent", [0, 1])
 def test_mesh_every_tiling(mesh_case, mode, kernel, persistent):
-    """Every rows x waves choice of set_tiling (both stages alike), one wave
-    per item and persistent, both stage kernels, on the 300 x 260 mesh of
+    """Every rows x waves choice of set_tiling (every kernel alike), one wave
+    per item and persistent, every step path, on the 300 x 260 mesh of
     3 x 2 uneven patches: 3 device-dt steps == the oracle."""
     m, U0, want = mesh_case
     sim = Sim2D.mesh(*MESH, mode=mode)
-    sim.set_stage_kernel(kernel)
+    configure(sim, kernel)
     for waves in autotune.WAVES:
         for rows in autotune.ROWS:
             print_case("mesh tiling", mode, kernel, persistent, waves, rows)
             sim.set_tiling(rows, waves, persistent)
             assert sim.get_tiling() == ((rows, rows), (waves, waves))
+            assert sim.get_step_tiling() == (rows, waves)
             _same(_run_mesh(sim, U0), want, mode, f"rows {rows} waves {waves}")
     sim.close()
 
@@ -112,12 +114,23 @@ def test_mesh_every_tiling(mesh_case, mode, kernel, persistent):
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_mesh_mixed_stage_tilings(mesh_case, mode):
     """Stage 1 and stage 2 with different rows and waves (the tuner sets them
-    per stage), on both stage kernels."""
+    per stage), on both two-launch paths; the one-launch step with its own
+    (stage 3) rows and waves."""
     m, U0, want = mesh_case
     rng = np.random.default_rng(11)
-    for kernel in ("auto", "register"):
+    sim = Sim2D.mesh(*MESH, mode=mode)
+    for trial in range(12):
+        r, wv = int(rng.choice(autotune.ROWS)), int(rng.choice(autotune.WAVES))
+        pers = int(trial % 2)
+        print_case("mesh step tiling", mode, (r, wv), pers)
+        sim.set_tiling(0, 0, pers)
+        sim.set_stage_tiling(3, r, wv)
+        assert sim.get_step_tiling() == (r, wv)
+        _same(_run_mesh(sim, U0), want, mode, f"step {(r, wv)} persistent {pers}")
+    sim.close()
+    for kernel in ("two_stage", "register"):
         sim = Sim2D.mesh(*MESH, mode=mode)
-        sim.set_stage_kernel(kernel)
+        configure(sim, kernel)
         for trial in range(12):
             r1, r2 = (int(x) for x in rng.choice(autotune.ROWS, 2))
             w1, w2 = (int(x) for x in rng.choice(autotune.WAVES, 2))
@@ -133,12 +146,15 @@ def test_mesh_mixed_stage_tilings(mesh_case, mode):
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 @pytest.mark.parametrize("persistent", [0, 1])
-def test_batch_every_tiling(batch_case, mode, persistent):
+@pytest.mark.parametrize("kernel", ["one_launch", "two_stage"])
+def test_batch_every_tiling(batch_case, mode, persistent, kernel):
     """The uneven batch (2 x 9 up to 256 x 256, one patch per problem) under
     every rows x waves choice, then per-problem rows (set_problem_rows, as
-    the tuner applies its per-size-class rows)."""
+    the tuner applies its per-size-class rows), for the one-launch step and
+    the two-launch step."""
     U0, refs = batch_case
     sim = Sim2D.batch(BATCH, mode=mode)
+    configure(sim, kernel)
     for waves in autotune.WAVES:
         for rows in autotune.ROWS:
             print_case("batch tiling", mode, persistent, waves, rows)
@@ -148,14 +164,15 @@ def test_batch_every_tiling(batch_case, mode, persistent):
     rng = np.random.default_rng(5)
     for trial in range(6):
         per = [[int(x) for x in rng.choice(autotune.ROWS, len(BATCH))] for _ in range(2)]
-        print_case("batch per-problem rows", mode, persistent, per)
+        print_case("batch per-problem rows", mode, persistent, kernel, per)
         sim.set_tiling(0, int(rng.choice(autotune.WAVES)), persistent)
         sim.set_problem_rows(1, per[0])
         sim.set_problem_rows(2, per[1])
+        sim.set_problem_rows(3, per[0])
         for k, (g, w) in enumerate(zip(_run_batch(sim, U0), refs)):
             _same(g, w, mode, f"problem {k}, per-problem rows {per}")
-    sim.set_problem_rows(1, None)
-    sim.set_problem_rows(2, None)
+    for st in (1, 2, 3):
+        sim.set_problem_rows(st, None)
     sim.close()
 
 
@@ -176,23 +193,29 @@ def test_autotuned_choice_is_parity_checked(mesh_case):
     sim.close()
 
 
-@pytest.mark.parametrize("tiling", [((12, 4), (12, 4)), ((12, 4), (8, 1)), ((32, 4), (48, 4))])
+@pytest.mark.parametrize("tiling", [("one_launch", (32, 4)), ("one_launch", (16, 1)), ("one_launch", (64, 2)),
+                                    ("two_stage", ((12, 4), (12, 4))), ("two_stage", ((12, 4), (8, 1)))])
 def test_sod4096_fast_device_dt_under_bench_tiling(oracle, tiling):
     """Full size, the workload bench.py times: 4096^2 planar Sod, 16 x 16
     patches of 256^2, outflow, fast numerics, device dt (start / advance:
     stage-2 speed slots, finalize, halo gathered in the kernels), under the
-    tilings the bench has run with ((rows, waves) of stage 1, stage 2).
+    tilings the bench runs with ((rows, waves) of the one-launch step, or of
+    stage 1 and stage 2).
     Five steps; after each, the GPU's dt is within 1e-14 of the oracle's
     compute_dt_2d of the same state, and the state within 1e-12 of the
     oracle's step from that state with the GPU's dt (the GPU state is fed
     back each step: lockstep)."""
     n, npt = 4096, 16
-    (r1, w1), (r2, w2) = tiling
+    path, t = tiling
     print_case("sod 4096^2 fast device dt", tiling)
     m = MeshSpec(n, n, npt, npt)
     sim = Sim2D.mesh(n, n, npt, npt, mode="fast")
-    sim.set_stage_tiling(1, r1, w1)
-    sim.set_stage_tiling(2, r2, w2)
+    configure(sim, path)
+    if path == "one_launch":
+        sim.set_stage_tiling(3, *t)
+    else:
+        sim.set_stage_tiling(1, *t[0])
+        sim.set_stage_tiling(2, *t[1])
     U = m.to_patches(P.sod2d(n, n))
     oracle.mesh_fill(U, m)
     worst = 0.0
@@ -215,14 +238,17 @@ def test_sod4096_fast_device_dt_under_bench_tiling(oracle, tiling):
     assert worst <= TOL_FAST, worst
 
 
-def test_ensemble512_persistent_launch(oracle):
+@pytest.mark.parametrize("kernel", ["one_launch", "two_stage"])
+def test_ensemble512_persistent_launch(oracle, kernel):
     """The 512-problem ensemble with the persistent launch (grid capped at
     the resident waves; each wave loops over the device work list), exact
     numerics, 3 device-dt steps from the captured step graph: bitwise the
     one-wave-per-item launch and, on a sample of problems, the oracle."""
-    print_case("ensemble 512 persistent")
+    print_case("ensemble 512 persistent", kernel)
     a, sizes, U0 = P.ensemble_batch(512, seed=3, mode="exact")
     b = Sim2D.batch(sizes, mode="exact")
+    configure(a, kernel)
+    configure(b, kernel)
     a.set_tiling(0, 0, 1)
     b.upload(U0)
     for s in (a, b):
