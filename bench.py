@@ -6,12 +6,16 @@ jitter, outflow boundaries, decomposed into 16 x 16 patches of 256^2 with
 one SSP-RK2 step = two cell-updates per cell (the reference's "Updates/sec
 (x2 for RK2)", examples/benchmark/benchmark_euler1d.cpp:217-218).
 
-Timed region: K steps of [dt finalize, fused stage 1, fused stage 2 + RK2
-combine + wave-speed reduction] (both stages gather their 2-cell halos from
-the neighbour patches on the device), launched one by one with no host round
-trip, inputs resident in HBM, bracketed by stream synchronisation. The same K
-steps replayed from the captured step graph give graph_value; a third pass
-with HIP events on the stream the kernels run on gives each kernel's
+Timed region: K steps of [dt finalize, one-launch RK2 step] (the step
+kernel runs both split stages in one strip march, keeps U1 on chip, gathers
+its 4-cell halo from the neighbour patches on the device and folds the wave
+speeds of the new state for the next dt), launched one by one with no host
+round trip, inputs resident in HBM, bracketed by stream synchronisation. It
+starts after the W warm-up steps and, if those took less than WARM_S
+seconds, after further untimed steps up to that time (the first launches
+after an upload run slower: clocks and address translation warm up). The
+same K steps replayed from the captured step graph give graph_value; a third
+pass with HIP events on the stream the kernels run on gives each kernel's
 duration for the roofline line.
 
 Prints ONE JSON line (rank 0, N=1).
@@ -32,6 +36,12 @@ METRIC = "FP64 Mcell-updates/s, 2D 4096² PLM+HLLE RK2; % HBM peak; adjoint/fwd 
 N, NP = 4096, 16
 BYTES_STAGE1 = 64.0  # read U^n, write U1 (4 vars x 8 B each)
 BYTES_STAGE2 = 96.0  # read U1, read U^n, write U^{n+1}
+# algorithmic bytes of one RK2 step per cell with the stage state U1
+# materialised (the metric's definition: stage 1 + stage 2 above); the
+# one-launch step kernel's compulsory traffic is read U^n + write U^{n+1}
+BYTES_STEP = BYTES_STAGE1 + BYTES_STAGE2
+BYTES_STEP_COMPULSORY = 64.0
+WARM_S = 1.5  # minimum warm-up time before the timed region
 
 
 def log(*a):
@@ -125,6 +135,7 @@ def ensemble_leg(steps: int, mode: str):
     from nest_amd import autotune, problems as P
 
     sim, sizes, U0 = P.ensemble_batch(512, seed=3, mode=mode)
+    sim_one = sim.one_launch
     cells = float(sum(a * b for a, b in sizes))
     sim.start(cfl=0.4)
     cfg = autotune.tune(sim, sizes, log=log)
@@ -150,7 +161,8 @@ def ensemble_leg(steps: int, mode: str):
     return {"problems": 512, "cells": int(cells), "steps": steps, "batched_mcups": round(batched, 1),
             "per_patch_mcups": round(per_patch, 1), "speedup": round(batched / per_patch, 2),
             "per_patch_steps_timed": pp_steps,
-            "tiling": {k: v for k, v in cfg.items() if k in ("1", "2", "persistent")}}
+            "step_kernel": "one launch per step" if sim_one else "two stage launches",
+            "tiling": {k: v for k, v in cfg.items() if k in ("1", "2", "3", "persistent")}}
 
 
 def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps: int = 3):
@@ -219,6 +231,21 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
         return run
 
     t_fwd = timed(fwd(K))
+    # the same K fixed-dt steps with the one-launch step kernel (nothing kept:
+    # the fastest forward of the map this library has)
+    from nest_amd import Sim2D, autotune
+    s1 = Sim2D.mesh(n, n, 4, 4, bcx="periodic", bcy="periodic", mode=mode)
+    U0p = s1.scatter(U0)
+    s1.upload(U0p)
+    s1.start(cfl=0.4, fixed_dt=dt)
+    autotune.tune(s1, sizes, log=log)
+
+    def fwd_one():  # (each run continues from the last state: the same work at a fixed dt)
+        s1.start(cfl=0.4, fixed_dt=dt)
+        s1.advance(K, graph=True)
+        s1.synchronize()
+    t_fwd_one = timed(fwd_one)
+    s1.close()
     t_store = timed(fwd(0))
     t_grad = timed(grad(0))
     kept = ro.rollout_interval() < 0
@@ -251,6 +278,9 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
     return {"workload": f"1024^2 periodic smooth state (4x4 patches of 256^2), K={K} fixed-dt RK2 steps, "
                         f"density L2 misfit, {storage}",
             "forward_ms": round(t_fwd * 1e3, 2), "forward_stored_ms": round(t_store * 1e3, 2),
+            "forward_one_launch_ms": round(t_fwd_one * 1e3, 2),
+            "adjoint_cost_vs_one_launch_forward": round(t_grad / t_fwd_one, 3),
+            "checkpointed_cost_vs_one_launch_forward": round(t_grad_ck / t_fwd_one, 3),
             "gradient_ms": round(t_grad * 1e3, 2), "adjoint_fwd_cost": round(t_grad / t_fwd, 3),
             "reverse_only_cost": round((t_grad - t_store) / t_fwd, 3),
             "checkpointed": {"interval": m_ck, "forward_ckpt_ms": round(t_ck * 1e3, 2),
@@ -395,8 +425,8 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
         "batching_speedup": round(t_s * nadj / t_b, 2)}}
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "traffic_stage2d_sod4096.json")
+def load_traffic(name: str):
+    path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f)
@@ -450,8 +480,14 @@ def main():
     sim.fill_halos()
     sim.start(cfl=0.4)
     log("bench: warmup")
+    tw = time.perf_counter()
     sim.advance(args.warmup)
     sim.synchronize()
+    extra_warm = 0
+    while time.perf_counter() - tw < WARM_S:  # steady state before the timed region
+        sim.advance(20)
+        sim.synchronize()
+        extra_warm += 20
 
     # the timed region: K device-dt steps launched one by one (the default
     # path; no graph, no events)
@@ -480,15 +516,24 @@ def main():
     log("bench: per-kernel timing")
     comp_ms, ev_total_ms = sim.advance_timed(args.steps)
     st = sim.status()
-    assert st["done"][0] == 0 and st["steps"][0] == args.warmup + 2 + 3 * args.steps
-    st1_ms = comp_ms[1] / args.steps
-    st2_ms = comp_ms[2] / args.steps
-    # dominant kernel: the stage-2 launch of the fused stage kernel (reads U1
-    # and U^n, writes U^{n+1}: 96 B per cell); the whole step (160 B per cell
-    # over the wall time) is reported beside it as step_hbm_frac
-    achieved = BYTES_STAGE2 * cells / (st2_ms * 1e-3) / 1e9
-    step_gbs = (BYTES_STAGE1 + BYTES_STAGE2) * cells / (ms_per_step * 1e-3) / 1e9
-    stages_gbs = (BYTES_STAGE1 + BYTES_STAGE2) * cells / ((st1_ms + st2_ms) * 1e-3) / 1e9
+    assert st["done"][0] == 0 and st["steps"][0] == args.warmup + extra_warm + 2 + 3 * args.steps
+    one = sim.one_launch
+    k1_ms = comp_ms[1] / args.steps
+    k2_ms = comp_ms[2] / args.steps
+    if one:
+        # dominant kernel: the one-launch step, 160 B per cell algorithmic
+        # (U1 materialised, as the metric counts it), 64 B compulsory
+        kernel_name = f"step2d_kernel<minmod> ({'fast' if args.mode == 'fast' else 'exact'})"
+        kern_ms = k1_ms
+        kern_bytes = BYTES_STEP
+    else:
+        # dominant kernel: stage 2 of the two-launch step (96 B per cell)
+        kernel_name = "stage2d_dma_kernel<minmod, 2>"
+        kern_ms = k2_ms
+        kern_bytes = BYTES_STAGE2
+    achieved = kern_bytes * cells / (kern_ms * 1e-3) / 1e9
+    step_gbs = BYTES_STEP * cells / (ms_per_step * 1e-3) / 1e9
+    stages_gbs = BYTES_STEP * cells / ((k1_ms + k2_ms) * 1e-3) / 1e9
 
     # plain per-patch drop-in path (host dt, one launch set per patch)
     pp = None
@@ -559,8 +604,12 @@ def main():
 
     copies = copy_peak_gbs()
     copy_gbs = copies["library_copy_kernel"]
-    traffic = load_traffic()
-    tr_bytes = (traffic or {}).get("stage2_bytes_per_launch")
+    if one:
+        traffic = load_traffic("traffic_step2d_sod4096.json")
+        tr_bytes = (traffic or {}).get("step_bytes_per_launch")
+    else:
+        traffic = load_traffic("traffic_stage2d_sod4096.json")
+        tr_bytes = (traffic or {}).get("stage2_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -579,9 +628,11 @@ def main():
                    "numerics": args.mode, "cells": N * N, "patches": NP * NP},
         "mcell_steps_per_s": round(value / 2.0, 1),
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-        "stages_hbm_frac": round(stages_gbs / HBM_PEAK_GBS, 4),
-        "kernel_ms": {"stage1": round(st1_ms, 4), "stage2": round(st2_ms, 4),
-                      "finalize": round(comp_ms[0] / args.steps, 4)},
+        "kernels_hbm_frac": round(stages_gbs / HBM_PEAK_GBS, 4),
+        "step_kernel": "one launch per step (U1 on chip)" if one else "two stage launches",
+        "kernel_ms": ({"step": round(k1_ms, 4)} if one else {"stage1": round(k1_ms, 4), "stage2": round(k2_ms, 4)})
+        | {"finalize": round(comp_ms[0] / args.steps, 4)},
+        "extra_warmup_steps": extra_warm,
         "tiling": tiling,
         "timed_path": "device-dt steps launched one by one (graph_value: the same steps replayed from the captured step graph)",
         "graph_value": round(graph_value, 1),
@@ -590,14 +641,17 @@ def main():
         "adjoint": adj,
         "kh_batched_adjoints": c5,
         "oned": oned,
-        "roofline": {"bound": "hbm", "kernel": "stage2d_dma_kernel<minmod, 2>", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes_per_launch": int(BYTES_STAGE2 * cells),
+                     "algorithmic_bytes_per_launch": int(kern_bytes * cells),
+                     "compulsory_bytes_per_launch": int((BYTES_STEP_COMPULSORY if one else kern_bytes) * cells),
+                     "compulsory_gbs": round((BYTES_STEP_COMPULSORY if one else kern_bytes) * cells
+                                             / (kern_ms * 1e-3) / 1e9, 1),
                      "measured_copy_peak": round(copy_gbs, 1), "frac_vs_copy_peak": round(achieved / copy_gbs, 4),
                      "copy_kernels": copies,
                      "traffic": tr_bytes,
                      # measured bytes (rocprof counters) moved per second of the launch
-                     "traffic_gbs": round(tr_bytes / (st2_ms * 1e-3) / 1e9, 1) if tr_bytes else None},
+                     "traffic_gbs": round(tr_bytes / (kern_ms * 1e-3) / 1e9, 1) if tr_bytes else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

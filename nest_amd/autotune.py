@@ -153,10 +153,20 @@ def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 5, cac
     return cfg
 
 
-def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = None, log=None):
+def tune(sim, sizes, arch: str = "gfx950", steps: int = 4, cache: str | None = None, log=None,
+         kernels: str = "step"):
     """Tune `sim` (already started: start() called) in place and return the
-    configuration; `sizes` = [(nx, ny)] per problem."""
+    configuration; `sizes` = [(nx, ny)] per problem. kernels = "step": the
+    kernels the handle's device-dt step runs (the one-launch step kernel
+    where it applies); "stages": the two stage kernels (what fixed-dt
+    rollouts and their recomputes run), timed through the two-launch step."""
     classes = size_classes(sizes)
+    if kernels == "stages":
+        sim.set_step_kernel("two_stage")
+        try:
+            return tune(sim, sizes, arch, steps, cache, log, kernels="step")
+        finally:
+            sim.set_step_kernel("auto")
     one = sim.one_launch
     key = ("step|" if one else "") + signature(sizes, classes, arch)
     path = _cache_path(cache)
