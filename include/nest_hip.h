@@ -113,6 +113,26 @@ int nest_hip_solver1d_start(nest_solver1d* s, double cfl, double t_end);
 int nest_hip_solver1d_advance(nest_solver1d* s, int nsteps);
 int nest_hip_solver1d_status(nest_solver1d* s, double* t, double* dt, int* steps, int* done);
 
+/* One device-dt SSP-RK2 step of a 1D state that allocates nothing:
+   U_out = step(U_in) (device buffers of 3*(n+4) doubles each, distinct;
+   U_out comes back halo-complete), with dt = min(cfl*dx/max(|u|+c) of U_in,
+   t_end - t) computed on the device and written to *dt_out (a device
+   double; NULL: not written). It runs the kernels of nest_hip_solver1d_start
+   + nest_hip_solver1d_advance(1) (the speeds of U_in, then the fused step
+   kernel, then the ghost fill), bit for bit the same result, but keeps
+   their speed slots, run state and parameters in a fixed device array of
+   the library (25.6 KB per GPU, held until the process ends) instead of
+   handle memory: no hipMalloc / hipFree and no host synchronisation, on
+   `stream` (NULL: the null stream). Every call on one GPU shares that
+   array, so calls that overlap on one GPU (several streams, several host
+   threads) race; serialise them. n is limited to 2^28 cells (32-bit
+   indexing); larger n returns NEST_ERR_INVALID. No counterpart in the
+   reference: for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops, harnesses that forbid allocation). */
+int nest_hip_euler1d_step_noalloc(const double* U_in, double* U_out, int n, double dx, double gamma, int bc,
+                                  int limiter, int mode, double cfl, double t, double t_end, double* dt_out,
+                                  void* stream);
+
 /* ---- reference-compatible entry points: exact signatures of reference
    src/backend_cuda/euler1d_solver.hpp:16-87 (periodic BC, minmod, halo 2,
    exact numerics). Errors print to stderr and return NULL / -1 / NaN

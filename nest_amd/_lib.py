@@ -54,6 +54,7 @@ SIGNATURES = {
     "nest_hip_solver1d_start": (_i, [_vp, _d, _d]),
     "nest_hip_solver1d_advance": (_i, [_vp, _i]),
     "nest_hip_solver1d_status": (_i, [_vp, _pd, _pd, _pi, _pi]),
+    "nest_hip_euler1d_step_noalloc": (_i, [_vp, _vp, _i, _d, _d, _i, _i, _i, _d, _d, _d, _vp, _vp]),
     "euler1d_gpu_create": (_vp, [_i, _i, _d, _d]),
     "euler1d_gpu_destroy": (None, [_vp]),
     "euler1d_gpu_upload": (None, [_vp, _pd]),

@@ -10,7 +10,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstddef>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1425,6 +1427,111 @@ int nest_hip_solver1d_status(nest_solver1d* s, double* t, double* dt, int* steps
     if (steps) *steps = s->run_h.steps;
     if (done) *done = s->run_h.done;
     if (s->run_h.done == 2) return fail(NEST_ERR_NONFINITE, "non-finite dt");
+    return NEST_OK;
+}
+
+// ---- one 1D step that allocates nothing -----------------------------------
+// The speed slots, run state and problem parameters of
+// nest_hip_solver1d_start + nest_hip_solver1d_advance live, for this entry
+// point, in a fixed device array of the library instead of handle memory:
+//   [0, 128)            nest_prob_state (cfl, t_end, dx of the call)
+//   [128, 256)          nest_run1d_state (t, dt, step count, done flag)
+//   [1024, 1024 + 3 S)  three speed-slot sets of S = NEST_SPEED_SLOTS *
+//                       NEST_SLOT_STRIDE u64, laid out as the handle's
+// Every GPU holds its own copy; calls that overlap on one GPU race on it.
+namespace {
+constexpr size_t kNoallocSet = (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE;
+constexpr size_t kNoallocRun = 128, kNoallocSlots = 1024;
+constexpr size_t kNoallocBytes = kNoallocSlots + 3 * kNoallocSet * sizeof(unsigned long long);
+static_assert(sizeof(nest_prob_state) <= kNoallocRun && kNoallocRun + sizeof(nest_run1d_state) <= kNoallocSlots,
+              "scratch layout");
+// the kernels index the state with 32-bit ints: 3 (n + 4) < 2^31
+constexpr int kNoallocMaxCells = 1 << 28;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of this GPU's copy of `scratch`, looked up once per device
+// (with that device current); the array's name is a host-side stand-in
+int scratch_address(unsigned char** out) {
+    int dev = -1;
+    NEST_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(NEST_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NEST_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_dev[dev]);
+    return NEST_OK;
+}
+
+// the state start() leaves: the call's parameters, time t, step 0, and all
+// three slot sets clear (a kernel, not a memset: callers may capture it)
+__global__ __launch_bounds__(256) void noalloc1d_init_kernel(unsigned char* base, nest_prob_state ps, double t) {
+    unsigned long long* slots = reinterpret_cast<unsigned long long*>(base + kNoallocSlots);
+    for (size_t i = threadIdx.x; i < 3 * kNoallocSet; i += blockDim.x) slots[i] = 0ull;
+    if (threadIdx.x == 0) {
+        *reinterpret_cast<nest_prob_state*>(base) = ps;
+        nest_run1d_state r{};
+        r.t[0] = t;
+        r.t[1] = t;
+        *reinterpret_cast<nest_run1d_state*>(base + kNoallocRun) = r;
+    }
+}
+}  // namespace
+
+int nest_hip_euler1d_step_noalloc(const double* U_in, double* U_out, int n, double dx, double gamma, int bc,
+                                  int limiter, int mode, double cfl, double t, double t_end, double* dt_out,
+                                  void* stream) {
+    if (!U_in || !U_out || U_in == U_out) return fail(NEST_ERR_INVALID, "need distinct U_in and U_out");
+    if (n < 2 || n > kNoallocMaxCells)
+        return fail(NEST_ERR_INVALID, "n must be in [2, " + std::to_string(kNoallocMaxCells) + "] (32-bit indexing)");
+    if (int rc = check_limiter_mode(limiter, mode)) return rc;
+    if (!(dx > 0.0) || !(gamma > 1.0) || bc < 0 || bc > 2 || !(cfl > 0.0))
+        return fail(NEST_ERR_INVALID, "need dx > 0, gamma > 1, bc in {0,1,2}, cfl > 0");
+    unsigned char* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const launch_table& L = table(mode);
+    nest_prob_state ps = make_prob(dx, 1.0, 1);
+    ps.t_end = t_end;
+    ps.cfl = cfl;
+    hipLaunchKernelGGL(noalloc1d_init_kernel, 1, 256, 0, st, base, ps, t);
+    unsigned long long* slots = reinterpret_cast<unsigned long long*>(base + kNoallocSlots);
+    stage1d_args a{};
+    a.U = const_cast<double*>(U_in);
+    a.U1 = U_out;
+    a.n = n;
+    a.ps = reinterpret_cast<nest_prob_state*>(base);
+    a.eos = make_eos(gamma);
+    a.host_dt = 0.0;
+    a.bc = bc;
+    // speeds of U_in into slot set 2, which step 0 reads its dt from
+    stage1d_args a0 = a;
+    a0.slots = slots + 2 * kNoallocSet;
+    L.speeds1d(a0, st);
+    // step 0 of nest_solver1d::device_step: U_in -> U_out
+    nest_run1d_state* run = reinterpret_cast<nest_run1d_state*>(base + kNoallocRun);
+    a.run = run;
+    a.par = 0;
+    a.slots = slots;
+    a.slots_rd = slots + 2 * kNoallocSet;
+    a.slots_clr = slots + kNoallocSet;
+    L.step1d(limiter, a, st);
+    L.fill1d(U_out, n, bc, nullptr, st);  // hand out a halo-complete state
+    if (dt_out)
+        NEST_HIP_TRY(hipMemcpyAsync(dt_out, reinterpret_cast<unsigned char*>(run) + offsetof(nest_run1d_state, dt),
+                                    sizeof(double), hipMemcpyDeviceToDevice, st));
+    NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
 
