@@ -48,19 +48,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(G, budget_s: float):
+def cpu_baseline(budget_s: float):
     """Oracle (CPU restatement of the reference numerics, OpenMP over patch
     rows) on the same 4096^2 workload, timed over whole RK2 steps until the
-    time budget is spent."""
-    import numpy as np
+    time budget is spent. It runs in a child process so that its OpenMP
+    runtime starts with OMP_PROC_BIND=close, OMP_PLACES=cores and one
+    thread per CPU this process may run on (torch has already started this
+    process's runtime)."""
+    import subprocess
 
+    threads = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(budget_s)],
+                       capture_output=True, text=True, env=env, timeout=max(120.0, 6 * budget_s))
+    if r.returncode != 0:
+        log("bench: cpu baseline failed: " + r.stderr[-2000:])
+        return None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    steps, el, nthr = res["steps"], res["seconds"], res["threads"]
+    val = N * N * 2.0 * steps / el / 1e6
+    return {"value": round(val, 3), "unit": "Mcell-updates/s", "cores": nthr, "kind": "port",
+            "sample": f"{steps} RK2 step(s) of the same 4096^2 16x16-patch Sod workload (dt from the "
+                      f"state each step), {el:.1f} s wall on {nthr} OpenMP threads (OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores) of {os.cpu_count()} CPUs ({cpu_model()}); oracle built -O3 "
+                      f"-ffp-contract=off for the generic x86-64 target"}
+
+
+def _cpu_baseline_child(budget_s: float) -> None:
+    """The timed oracle loop of cpu_baseline (child process)."""
+    from nest_amd import problems as P
     from oracle.oracle import MeshSpec, Oracle
 
     o = Oracle()
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    o.set_threads(threads)
     m = MeshSpec(N, N, NP, NP)
-    U = m.to_patches(G)
+    U = m.to_patches(P.sod2d(N, N, jitter=1e-3, seed=2))
     o.mesh_fill(U, m)
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -70,13 +91,7 @@ def cpu_baseline(G, budget_s: float):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 20:
             break
-    val = N * N * 2.0 * steps / el / 1e6
-    del U
-    _ = np
-    return {"value": round(val, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} RK2 step(s) of the same 4096^2 16x16-patch Sod workload (dt from the "
-                      f"state each step), {el:.1f} s wall on {threads} OpenMP threads "
-                      f"of {os.cpu_count()} visible ({cpu_model()})"}
+    print(json.dumps({"steps": steps, "seconds": el, "threads": o.get_threads()}), flush=True)
 
 
 def copy_peak_gbs(gib: int = 2, reps: int = 10) -> dict:
@@ -434,6 +449,9 @@ def load_traffic(name: str):
 
 
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--cpu-baseline-child":
+        _cpu_baseline_child(float(sys.argv[2]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -583,7 +601,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(G, args.cpu_budget)
+        cpu = cpu_baseline(args.cpu_budget)
 
     if args.copy_probe_mib > 0:
         # a plain device copy of known size (the library's copy kernel), for
