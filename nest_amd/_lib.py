@@ -49,6 +49,7 @@ SIGNATURES = {
     "nest_hip_solver1d_rk2_step": (_i, [_vp, _d]),
     "nest_hip_solver1d_run": (_i, [_vp, _d, _d, _pi]),
     "nest_hip_solver1d_set_stream": (_i, [_vp, _vp]),
+    "nest_hip_solver1d_set_step_kernel": (_i, [_vp, _i]),
     "nest_hip_solver1d_stage": (_i, [_vp, _i, _d]),
     "nest_hip_solver1d_exchange": (_i, [_vp, _i]),
     "nest_hip_solver1d_start": (_i, [_vp, _d, _d]),

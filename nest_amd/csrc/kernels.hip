@@ -1442,6 +1442,128 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     }
 }
 
+// The same whole 1D RK2 step with no LDS tile and no barrier inside the
+// step: every wave works on its own. Lane l of a segment holds cell
+// g0 - 4 + l; neighbour values move by DPP wave shifts (the Y-sweep
+// machinery of the 2D kernels on 3 variables): stage 1 is valid on lanes
+// 2..61, stage 2 on lanes 4..59, so a segment produces 56 cells. Past an
+// outflow / reflecting wall, U1 comes from the lane of the mapped cell
+// (nest_step_lane_source), momentum negated at a reflecting wall; past a
+// periodic end stage 1 sees the stencil of the cell it wraps to. Each wave
+// runs a.segs consecutive segments (one dt read per wave); its speed maximum
+// meets the block's in LDS and reaches the slots with one atomic per block.
+// Arithmetic and order per cell are those of step1d_kernel.
+template <int LIM>
+__global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
+    __shared__ unsigned long long wmax[4];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wib);
+    const int n = a.n, ne = n + 4;
+    // issued first, used after the loads: the run's flag, this step's time,
+    // the previous step's speeds
+    const int done_in = a.run->done;
+    const double t_in = a.run->t[a.par];
+    const unsigned long long v_in = lane < kStepSlots ? a.slots_rd[(size_t)lane * NEST_SLOT_STRIDE] : 0ull;
+    const unsigned long long v = wave_max_u64(v_in);
+    // the step's dt (schedule_step's arithmetic), the same in every wave
+    const nest_prob_state& P = *a.ps;
+    const double sx = __longlong_as_double((long long)v);
+    int done = 0;
+    double dt = 0.0;
+    if (done_in) {
+        done = -1;  // already finished: leave the run's state alone
+    } else if (!(t_in < P.t_end)) {
+        done = 1;
+    } else {
+        dt = P.fixed_dt > 0.0 ? P.fixed_dt : P.cfl * P.dx / sx;
+        dt = dmin(dt, P.t_end - t_in);
+        if (!(dt > 0.0) || dt != dt || dt == __builtin_inf() || sx != sx) done = 2;
+    }
+    if (wid == 0) {
+        if (lane == 0 && done >= 0) {
+            if (done) {
+                a.run->done = done;
+                if (done == 2) a.run->dt = dt;
+            } else {
+                a.run->dt = dt;
+                a.run->t[a.par ^ 1] = t_in + dt;
+                a.run->steps += 1;
+            }
+        }
+        if (lane < kStepSlots) a.slots_clr[(size_t)lane * NEST_SLOT_STRIDE] = 0ull;  // for the next step
+    }
+    const bool go = !done;
+    const double dtdx = dt / P.dx;
+    const double* in = a.U;
+    double* out = a.U1;
+    unsigned long long sm = 0;
+    for (int sg = 0; sg < a.segs; ++sg) {
+        const int g0 = (wid * a.segs + sg) * 56;
+        if (g0 >= n) break;
+        const int g = g0 - 4 + lane;
+        const bool store = lane >= 4 && lane < 60 && g < n;
+        if (!go) {
+            // a step past the end passes the state through
+            if (store) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) out[k * ne + g + 2] = in[k * ne + g + 2];
+            }
+            continue;
+        }
+        const int src = nest_map_index(g, n, a.bc) + 2;
+        const bool outside = g < 0 || g >= n;
+        const double m = in[ne + src];
+        double c0 = in[src], c1 = a.bc == 2 && outside ? -m : m, c2 = in[2 * ne + src];
+        const double z0 = c0, z1 = c1, z2 = c2;  // U^n of this lane's cell (the combine)
+        // one stage on the lanes: c <- c - dtdx (F(l+1/2) - F(l-1/2))
+        auto stage = [&](double& u0, double& u1, double& u2) {
+            prim4 W = prim1d(u0, u1, u2, a.eos);
+            const double d0 = W.r - lane_up_z(W.r), d1 = W.u - lane_up_z(W.u), d2 = W.p - lane_up_z(W.p);
+            const prim4 dl{d0, d1, d2, 0.0};
+            const prim4 dr{lane_down_z(d0), lane_down_z(d1), lane_down_z(d2), 0.0};
+            const prim4 S = slope4<LIM>(dl, dr);
+            const prim4 Rm = face_minus(W, S);
+            const prim4 Rf{lane_down_z(Rm.r), lane_down_z(Rm.u), lane_down_z(Rm.p), 0.0};
+            double f[4];
+            face_flux<false>(face_plus(W, S), Rf, a.eos, f);
+            u0 = u0 - dtdx * (f[0] - lane_up_z(f[0]));
+            u1 = u1 - dtdx * (f[1] - lane_up_z(f[1]));
+            u2 = u2 - dtdx * (f[2] - lane_up_z(f[2]));
+        };
+        stage(c0, c1, c2);
+        if (a.bc != 0 && (g0 - 4 < 0 || g0 + 59 >= n)) {
+            // U1 past an outflow / reflecting wall from the lane of the mapped cell
+            const int sl = nest_step_lane_source(lane, g, n, a.bc);
+            c0 = __shfl(c0, sl);
+            c1 = __shfl(c1, sl);
+            c2 = __shfl(c2, sl);
+            if (a.bc == 2 && outside) c1 = -c1;
+        }
+        stage(c0, c1, c2);
+        if (store) {
+            const double o0 = 0.5 * (z0 + c0), o1 = 0.5 * (z1 + c1), o2 = 0.5 * (z2 + c2);
+            const prim4 w = prim1d(o0, o1, o2, a.eos);
+            const double sp = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
+            const unsigned long long b = (unsigned long long)__double_as_longlong(sp);
+            sm = b > sm ? b : sm;
+            const int e = g + 2;
+            out[e] = o0;
+            out[ne + e] = o1;
+            out[2 * ne + e] = o2;
+        }
+    }
+    if (!go) return;
+    sm = wave_max_u64(sm);
+    if (lane == 0) wmax[wib] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = b > wmax[w] ? b : wmax[w];
+        atomicMax(a.slots + (size_t)(blockIdx.x & (kStepSlots - 1)) * NEST_SLOT_STRIDE, b);
+    }
+}
+
 __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
     __shared__ unsigned long long wmax[4];
     const int n = a.n, ne = n + 4;
@@ -1573,6 +1695,16 @@ void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s) {
 }
 
 void launch_step1d(int lim, const stage1d_args& a, hipStream_t s) {
+    if (a.segs > 0) {  // lane kernel: a.segs segments of 56 cells per wave, 4 waves per block
+        const int waves = (a.n + 56 * a.segs - 1) / (56 * a.segs);
+        const int grid = (waves + 3) / 4;
+        switch (lim) {
+            case LIM_MC: hipLaunchKernelGGL((step1d_lanes_kernel<LIM_MC>), grid, 256, 0, s, a); break;
+            case LIM_NONE: hipLaunchKernelGGL((step1d_lanes_kernel<LIM_NONE>), grid, 256, 0, s, a); break;
+            default: hipLaunchKernelGGL((step1d_lanes_kernel<LIM_MINMOD>), grid, 256, 0, s, a); break;
+        }
+        return;
+    }
     const int grid = (a.n + 255) / 256;
     switch (lim) {
         case LIM_MC: hipLaunchKernelGGL((step1d_kernel<LIM_MC>), grid, 256, 0, s, a); break;

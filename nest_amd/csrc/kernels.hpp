@@ -90,6 +90,9 @@ struct stage1d_args {
     const unsigned long long* slots_rd;
     unsigned long long* slots_clr;
     int par;
+    // fused step kernel: 0 the LDS-tile kernel (256 cells per block); > 0
+    // the lane kernel with this many 56-cell segments per wave
+    int segs;
 };
 
 NEST_DECLARE_LAUNCHERS(nest_exact)

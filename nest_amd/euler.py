@@ -70,6 +70,11 @@ class Solver1D:
         from . import dat
         return dat.write_sod1d(directory, self.download().reshape(3, -1)[:, 2:-2], time, step, x_min, x_max, gamma)
 
+    def set_step_kernel(self, kind: str = "auto") -> None:
+        """Kernel of the fused device-dt steps: "auto", "tile" (LDS tile per
+        block) or "lanes" (each wave on its own 56-cell segments)."""
+        check(self.L.nest_hip_solver1d_set_step_kernel(self.h, {"auto": 0, "tile": 1, "lanes": 2}[kind]))
+
     def stage(self, stage: int, dt: float) -> None:
         """One SSP-RK2 stage with the given dt (1: U1 = S(U); 2: U = (U + S(U1)) / 2)."""
         check(self.L.nest_hip_solver1d_stage(self.h, stage, dt))

@@ -218,8 +218,29 @@ static void check_step1d_wall() {
             }
 }
 
+// The 1D lane kernel's wall copies: in every 56-cell segment, every lane
+// stage 2 reads (2..61) whose cell lies past an outflow / reflecting wall
+// but within two cells of it takes U1 from a lane stage 1 computes (2..61)
+// holding the mapped cell; lanes inside keep their own.
+static void check_step1d_lanes() {
+    for (int bc = 1; bc <= 2; ++bc)
+        for (int n = 2; n <= 1100; ++n)
+            for (int g0 = 0; g0 < n; g0 += 56)
+                for (int lane = 2; lane < 62; ++lane) {
+                    const int g = g0 - 4 + lane;
+                    const int s = nest_step_lane_source(lane, g, n, bc);
+                    if (g >= 0 && g < n) {
+                        CHECK(s == lane, "lanes identity n=%d g0=%d lane=%d", n, g0, lane);
+                    } else if (g >= -2 && g <= n + 1) {
+                        CHECK(s >= 2 && s < 62 && g0 - 4 + s == nest_map_index(g, n, bc),
+                              "lanes source n=%d g0=%d lane=%d bc=%d s=%d", n, g0, lane, bc, s);
+                    }
+                }
+}
+
 int main() {
     check_step1d_wall();
+    check_step1d_lanes();
     check_1d_gather();
     for (int bcx = 0; bcx < 3; ++bcx)  // periodic, outflow, reflecting
         for (int bcy = 0; bcy < 3; ++bcy) {

@@ -57,18 +57,23 @@ def test_1d_sod_lockstep(oracle, mode, bc, lim):
     assert worst <= TOL_FAST
 
 
+STEP1D = ["tile", "lanes"]
+
+
+@pytest.mark.parametrize("kernel", STEP1D)
 @pytest.mark.parametrize("lim", LIMITERS)
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-def test_1d_sod_run_to_t_end(oracle, mode, lim):
+def test_1d_sod_run_to_t_end(oracle, mode, lim, kernel):
     """1D Sod, N=1000, to t = 0.2 with device-side dt (the fused one-launch
-    step), each limiter: step count and final state."""
+    step, both kernels), each limiter: step count and final state."""
     n, cfl, t_end = 1000, 0.4, 0.2
-    print_case("1d run", mode, lim)
+    print_case("1d run", mode, lim, kernel)
     U = P.sod1d(n)
     oracle.fill_1d(U, n, bc="outflow")
     ref = U.copy()
     k_ref, _ = oracle.run_1d(ref, n, t_end, cfl, 1.0 / n, bc="outflow", lim=lim)
     s = Solver1D(n, 1.0 / n, bc="outflow", mode=mode, limiter=lim)
+    s.set_step_kernel(kernel)
     s.upload(U)
     k = s.run(t_end, cfl)
     got = s.download().reshape(3, -1)[:, 2:-2]
@@ -83,9 +88,10 @@ def test_1d_sod_run_to_t_end(oracle, mode, lim):
     assert abs(got[0].sum() - want[0].sum()) <= 1e-12 * want[0].sum()
 
 
+@pytest.mark.parametrize("kernel", STEP1D)
 @pytest.mark.parametrize("lim", LIMITERS)
 @pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
-def test_1d_run_fused_step_sizes(oracle, bc, lim):
+def test_1d_run_fused_step_sizes(oracle, bc, lim, kernel):
     """Solver1D.run takes each device-dt step in one fused launch that keeps
     U1 in LDS, alternating the state between two buffers. Bitwise equal to
     the oracle's run (exact numerics) for sizes of 2 and 3 cells (one block
@@ -96,7 +102,7 @@ def test_1d_run_fused_step_sizes(oracle, bc, lim):
     # tube (on the strong jump at 2-3 cells the unlimited scheme drives the
     # density negative, and the run ends with a non-finite dt on either side)
     left, right = ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1)) if lim == "minmod" else ((1.0, 0.0, 1.0), (0.8, 0.0, 0.75))
-    for n, t_probe in [(2, 0.5), (3, 0.62), (257, 0.05), (1000, 0.05)]:
+    for n, t_probe in [(2, 0.5), (3, 0.62), (57, 0.1), (257, 0.05), (1000, 0.05)]:
         U = P.sod1d(n, left=left, right=right)
         oracle.fill_1d(U, n, bc=bc)
         # end times half-way into steps j and j + 1 of the oracle's dt
@@ -107,10 +113,11 @@ def test_1d_run_fused_step_sizes(oracle, bc, lim):
         j = k0 - 3
         parities = set()
         for t_end in (T[j] + 0.5 * dts[j], T[j + 1] + 0.5 * dts[j + 1]):
-            print_case("1d run fused", n, bc, lim, t_end)
+            print_case("1d run fused", n, bc, lim, kernel, t_end)
             ref = U.copy()
             k_ref, _ = oracle.run_1d(ref, n, t_end, 0.4, 1.0 / n, bc=bc, lim=lim)
             s = Solver1D(n, 1.0 / n, bc=bc, mode="exact", limiter=lim)
+            s.set_step_kernel(kernel)
             s.upload(U)
             k = s.run(t_end, 0.4)
             got = s.download()
@@ -606,8 +613,9 @@ def test_dat_files_of_gpu_states(oracle, tmp_path):
         assert open(a, "rb").read() == open(b, "rb").read(), a
 
 
+@pytest.mark.parametrize("kernel", STEP1D)
 @pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
-def test_1d_start_advance_and_stages(oracle, bc):
+def test_1d_start_advance_and_stages(oracle, bc, kernel):
     """The 1D solver by step count (start / advance, device dt, one fused
     launch per step) in calls of 3, 4 and 1 steps: each call leaves the state
     in U whatever the parity of the steps taken, and equals the oracle's
@@ -616,10 +624,11 @@ def test_1d_start_advance_and_stages(oracle, bc):
     (stage 1, exchange of U1, stage 2) equal one oracle RK2 step."""
     n, cfl = 400, 0.4
     dx = 1.0 / n
-    print_case("1d start/advance/stages", bc)
+    print_case("1d start/advance/stages", bc, kernel)
     U = P.sod1d(n)
     oracle.fill_1d(U, n, bc=bc)
     s = Solver1D(n, dx, bc=bc, mode="exact")
+    s.set_step_kernel(kernel)
     s.upload(U)
     s.start(cfl)
     done, t, ref = 0, 0.0, U.copy()
@@ -656,3 +665,35 @@ def test_1d_start_advance_and_stages(oracle, bc):
     oracle.fill_1d(ref, n, bc=bc)
     assert np.array_equal(s.download(), ref)
     s.close()
+
+
+@pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
+def test_1d_large_lane_kernel_segments(oracle, bc):
+    """The lane kernel with several segments per wave (n = 56 * 16384 + 77:
+    2 segments per wave, a ragged last wave and block), fast numerics: 4
+    device-dt steps equal the tile kernel's within 1e-12 and the oracle's
+    (exact numerics, bitwise)."""
+    n = 56 * 16384 + 77
+    print_case("1d lanes segments", bc)
+    U = P.sod1d(n)
+    oracle.fill_1d(U, n, bc=bc)
+    ref = U.copy()
+    dx = 1.0 / n
+    for _ in range(4):
+        dt = 0.4 * dx / oracle.max_speed_1d(ref, n)
+        oracle.rk2_step_1d(ref, n, dt, dx, bc=bc)
+    oracle.fill_1d(ref, n, bc=bc)
+    outs = {}
+    for mode in ("exact", "fast"):
+        for kernel in ("lanes", "tile"):
+            s = Solver1D(n, dx, bc=bc, mode=mode)
+            s.set_step_kernel(kernel)
+            s.upload(U)
+            s.start(0.4)
+            s.advance(4)
+            outs[(mode, kernel)] = s.download()
+            s.close()
+    assert np.array_equal(outs[("exact", "lanes")], ref)
+    assert np.array_equal(outs[("exact", "tile")], ref)
+    got, want = outs[("fast", "lanes")].reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2]
+    assert rel_linf_vars(got, want) <= 4e-12
