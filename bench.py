@@ -313,15 +313,17 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
 
 
 def oned_leg(large_steps: int = 200, sod_t_end: float = 0.2):
-    """1D Sod with device dt through Solver1D.run (one launch per step, dt
-    included, replayed 48 steps at a time from a captured graph):
+    """1D Sod with device dt through Solver1D.run (one launch of the lane step
+    kernel per step, dt included, replayed 48 steps at a time from a captured
+    graph):
     * the configuration of the reference's published 1D GPU figure
       (examples/benchmark/benchmark_euler1d.cpp: N = 1e5, periodic, CFL 0.4,
       t = 0.2; exact numerics, as the euler1d_gpu_* drop-in), median of 3 runs
       after one untimed run (`sod_t_end` shortens it for profiling runs);
-    * N = 2^24 cells (outflow, fast numerics), about `large_steps` steps: the
-      stage kernels' bandwidth, 120 B per cell-step (stage 1 reads and writes
-      3 doubles, stage 2 reads 6 and writes 3)."""
+    * N = 2^24 cells (outflow, fast numerics), about `large_steps` steps,
+      counted at the algorithmic 120 B per cell-step (the two stages with U1
+      materialised: stage 1 reads and writes 3 doubles, stage 2 reads 6 and
+      writes 3; the fused kernel itself reads Un and writes Un+1, 48 B)."""
     from nest_amd import Solver1D, problems as P
     out = {}
     n = 100000

@@ -1460,8 +1460,24 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
     const int wib = threadIdx.x >> 6;
     const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wib);
     const int n = a.n, ne = n + 4;
-    // issued first, used after the loads: the run's flag, this step's time,
-    // the previous step's speeds
+    const double* in = a.U;
+    double* out = a.U1;
+    // U^n of segment sg's lanes (cell g0 - 4 + lane through the boundary map;
+    // momentum not yet negated past a reflecting wall)
+    double p0 = 0.0, pm = 0.0, p2 = 0.0;
+    auto fetch = [&](int sg) {
+        const int g0 = (wid * a.segs + sg) * 56;
+        if (sg < a.segs && g0 < n) {
+            const int src = nest_map_index(g0 - 4 + lane, n, a.bc) + 2;
+            p0 = in[src];
+            pm = in[ne + src];
+            p2 = in[2 * ne + src];
+        }
+    };
+    // the first segment's loads go out before the dt arithmetic, which they
+    // do not depend on; then the run's flag, this step's time, the previous
+    // step's speeds
+    fetch(0);
     const int done_in = a.run->done;
     const double t_in = a.run->t[a.par];
     const unsigned long long v_in = lane < kStepSlots ? a.slots_rd[(size_t)lane * NEST_SLOT_STRIDE] : 0ull;
@@ -1495,26 +1511,28 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
     }
     const bool go = !done;
     const double dtdx = dt / P.dx;
-    const double* in = a.U;
-    double* out = a.U1;
     unsigned long long sm = 0;
     for (int sg = 0; sg < a.segs; ++sg) {
         const int g0 = (wid * a.segs + sg) * 56;
         if (g0 >= n) break;
         const int g = g0 - 4 + lane;
         const bool store = lane >= 4 && lane < 60 && g < n;
+        const double m = pm;
+        double c0 = p0, c2 = p2;
+        fetch(sg + 1);  // the next segment's loads, in flight during this one
         if (!go) {
-            // a step past the end passes the state through
+            // a step past the end passes the state through (stored lanes
+            // hold interior cells: their loads are the cells themselves)
             if (store) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) out[k * ne + g + 2] = in[k * ne + g + 2];
+                const int e = g + 2;
+                out[e] = c0;
+                out[ne + e] = m;
+                out[2 * ne + e] = c2;
             }
             continue;
         }
-        const int src = nest_map_index(g, n, a.bc) + 2;
         const bool outside = g < 0 || g >= n;
-        const double m = in[ne + src];
-        double c0 = in[src], c1 = a.bc == 2 && outside ? -m : m, c2 = in[2 * ne + src];
+        double c1 = a.bc == 2 && outside ? -m : m;
         const double z0 = c0, z1 = c1, z2 = c2;  // U^n of this lane's cell (the combine)
         // one stage on the lanes: c <- c - dtdx (F(l+1/2) - F(l-1/2))
         auto stage = [&](double& u0, double& u1, double& u2) {
