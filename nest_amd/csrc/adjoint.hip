@@ -31,13 +31,13 @@
 // Minimum waves per SIMD requested for the Y and X kernels (register budget
 // 512/N per lane). Buffers are addressed by 32-bit element offsets (below
 // 2^32 elements, checked by the host), which keeps the register counts down.
-// Y kernel: 3 waves per SIMD in fast numerics with the minmod / none
-// limiters (166 VGPRs); exact numerics (IEEE divisions) and the MC limiter
-// (coefficient state) need more registers and stay at 2. X kernel: 2 (its
-// interior runs two march positions per trip; one position per trip fits 3
-// waves in the periodic fast instances but measured 4% slower overall).
+// Y kernel: 2 (with the cotangent row loaded one iteration ahead, the fast
+// minmod / none instances need 168+ VGPRs, past the 3-wave budget). X
+// kernel: 2 (its interior runs two march positions per trip; one position
+// per trip fits 3 waves in the periodic fast instances but measured 4%
+// slower overall).
 #ifndef NEST_ADJY_WAVES
-#define NEST_ADJY_WAVES(LIM) (NEST_FAST && (LIM) != LIM_MC ? 3 : 2)
+#define NEST_ADJY_WAVES(LIM) 2
 #endif
 #ifndef NEST_ADJX_WAVES
 #define NEST_ADJX_WAVES 2
@@ -255,11 +255,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         // Y-sweep reverse of row i (the two do not depend on each other);
         // past the strip's last row it computes a row that is discarded
         // (its loads are clamped to the patch)
-        double vc[4];
+        // the cotangent row of the next iteration is loaded one iteration
+        // ahead (unconditionally: row r1 is at most the patch's first ghost
+        // row, in bounds)
+        double vc[4], Gn[4];
+        load_int(a.G, w.r0, Gn);
         xstep(w.r0, vc);
         for (int i = w.r0; i < w.r1; ++i) {
             double A[4];
-            load_int(a.G, i, A);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) A[k] = Gn[k];
+            load_int(a.G, i + 1, Gn);
             double vn[4];
             xstep(i + 1, vn);
             yrev(i, vc, A);
@@ -327,24 +333,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
             const int gr = pd.x0 + t;
             return ring || (gr >= 0 && gr < G.NX);
         };
-        // out(row) = Vb(row) + (dW/dU)^T Wbar (+ h H(row)); Vb(row) is re-read
-        // here (it was loaded two steps earlier, so it is cache-resident)
-        // rather than carried in registers
-        auto emit = [&](int row, const prim4& Wp, const prim4& Wbar) {
+        // out(row) = Vb(row) + (dW/dU)^T Wbar (+ h H(row)), in two halves: the
+        // loads of Vb(row) and H(row) (emit_load; Vb(row) was loaded two
+        // positions earlier, so it is cache-resident), issued before the
+        // position's face reverse, and the sum and store (emit_store)
+        // (loads without branches, so that the compiler's wait counts stay
+        // exact: without H the second half re-reads Vb and is not used)
+        const double* const Hs = a.H ? a.H : a.Vb;
+        auto emit_load = [&](int row, double* vb, double* hh) {
+            const unsigned o = col + (unsigned)(row + 2) * (unsigned)snye;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) vb[k] = a.Vb[o + (unsigned)k * (unsigned)svs];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) hh[k] = Hs[o + (unsigned)k * (unsigned)svs];
+        };
+        auto emit_store = [&](int row, const prim4& Wp, const prim4& Wbar, const double* vb, const double* hh) {
             double Ub[4];
             prim2d_rev(Wp, true, Wbar, a.eos, Ub);
             const unsigned o = col + (unsigned)(row + 2) * (unsigned)snye;
             double r[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = a.Vb[o + (unsigned)k * (unsigned)svs] + Ub[k];
+            for (int k = 0; k < 4; ++k) r[k] = vb[k] + Ub[k];
             if (a.H) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) r[k] += a.h * a.H[o + (unsigned)k * (unsigned)svs];
+                for (int k = 0; k < 4; ++k) r[k] += a.h * hh[k];
             }
             if (own) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) a.out[o + (unsigned)k * (unsigned)svs] = r[k];
             }
+        };
+        auto emit = [&](int row, const prim4& Wp, const prim4& Wbar) {
+            double vb[4], hh[4];
+            emit_load(row, vb, hh);
+            emit_store(row, Wp, Wbar, vb, hh);
         };
 
         // The first face that reaches an emitted cell is r0 - 3/2 (through
@@ -365,14 +387,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
         if (interior(w.r0 - 2)) load_pos(a.Vb, w.r0 - 2, Vb2);
         prim4 Wb3 = zero4(), Wb2 = zero4(), sb2 = zero4(), acc = zero4();
         prim4 s2 = slope4<LIM>(sub4(W2, W3), sub4(W1, W2));  // slope of cell t-2, carried
+        // the operands of march position t, U(t) and Vb(t-1), are loaded one
+        // position ahead (software pipeline: they land while the previous
+        // position's face is reversed). The loads are unconditional (any
+        // position resolves through the boundary map, in bounds); Vb of a
+        // ghost row past an outflow end is then taken as 0.
+        double nu[4], nvb[4];
+        auto prefetch = [&](int t) {
+            load_pos(a.U, t, nu);
+            load_pos(a.Vb, t - 1, nvb);
+            if (OUTX) {
+                const bool in = interior(t - 1);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) nvb[k] = in ? nvb[k] : 0.0;
+            }
+        };
+        prefetch(w.r0);
         // one march position: face t - 3/2 reversed, the cotangent of cell
         // t - 3 completed in Wb3
         auto body = [&](int t) {
-            double uu[4];
-            load_pos(a.U, t, uu);
+            double uu[4], Vb1[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uu[k] = nu[k];
+                Vb1[k] = nvb[k];
+            }
+            prefetch(t + 1);
             const prim4 Wt = prim2d(uu[0], uu[1], uu[2], uu[3], true, a.eos);
-            double Vb1[4] = {0.0, 0.0, 0.0, 0.0};
-            if (!OUTX || interior(t - 1)) load_pos(a.Vb, t - 1, Vb1);
             const prim4 d3 = sub4(W2, W3), d2 = sub4(W1, W2), d1 = sub4(Wt, W1);
             const prim4 s1 = slope4<LIM>(d2, d1);
             const limsel4<LIM> cf = limiter_sel4<LIM>(d3, d2);
@@ -433,8 +474,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
         // positions per trip so that the two face reverses, which do not
         // depend on each other, can be interleaved
         auto fast = [&](int t) {
+            double vb[4], hh[4];
+            emit_load(t - 3, vb, hh);
             auto shift = body(t);
-            emit(t - 3, W3, Wb3);
+            emit_store(t - 3, W3, Wb3, vb, hh);
             shift();
         };
         const int ta = w.r0 + 3 + (lo ? 1 : 0);  // first position of the interior part
