@@ -283,21 +283,23 @@ int nest_hip_sim2d_adjoint_stage(nest_sim2d* s, const double* U, const double* G
                                  double h, double* out, double dt);
 /* Ubar <- (dU'/dU)^T Ubar for one RK2 step of size dt taken from state U0. */
 int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double dt);
-/* nsteps fixed-dt RK2 steps of the handle's state, storing a checkpoint every
-   `interval` steps (0: ceil(sqrt(nsteps))) in device buffers owned by the
-   handle; the final state's halos are refilled. With two or more intervals,
-   the states of the last interval and their stage-1 states are kept too (in
-   the buffers the reverse sweep reads), so its reverse needs no recompute.
-   interval < 0 keeps every state and stage-1 state (2 nsteps states of
-   device memory): the reverse then recomputes nothing. interval >= nsteps is
-   a plain forward that keeps only the initial state. Buffers are kept by the
-   handle (grown, never shrunk) until it is destroyed. */
+/* nsteps fixed-dt RK2 steps of the handle's state in checkpoint intervals of
+   `interval` steps (0: ceil(sqrt(nsteps))), the first interval taking the
+   remainder; the first state of every interval is stored as its checkpoint
+   in device buffers owned by the handle, and the final state's halos are
+   refilled. With two or more intervals, the states of the last four
+   intervals and their stage-1 states are kept too (in the four buffer sets
+   the reverse sweep reads), so their reverse needs no recompute. interval <
+   0 keeps every state and stage-1 state (2 nsteps states of device memory):
+   the reverse then recomputes nothing. interval >= nsteps is a plain forward
+   that keeps only the initial state. Buffers are kept by the handle (grown,
+   never shrunk) until it is destroyed. */
 int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval);
 /* Reverse of the last rollout: Ubar (in: cotangent of the final state, out:
    cotangent of the initial state), recomputing each checkpoint interval's
    states (and their stage-1 states) from its checkpoint; the recompute of
    one interval runs on an auxiliary stream while the next is reversed.
-   Memory: ceil(K/m) + 2 (2m - 1) + 3 states (2K + 3 with every state kept). */
+   Memory: ceil(K/m) + 4 (2m - 1) + 3 states (2K + 3 with every state kept). */
 int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar);
 int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval, double* dt);
 /* advance() without a graph, with HIP events recorded on the handle's stream

@@ -7,10 +7,11 @@ library's hand-written discrete adjoint (nest_hip_sim2d_rollout_adjoint),
 which reverses the stored rollout. By default every state of the rollout
 is kept in device memory when 2K states fit in half of the free memory (the
 reverse then recomputes nothing); otherwise it goes interval by interval,
-recomputing each interval's states from its checkpoint (checkpoints every
-ceil(sqrt(K)) steps). Derivatives are exact for the discrete map, branch by
-branch (limiter cases, HLLE upwind / subsonic branch, Davis speeds, floors,
-the periodic / zero-gradient halo fill).
+recomputing each interval's states from its checkpoint (intervals of
+ceil(sqrt(K)) steps; the forward keeps the states of the last four).
+Derivatives are exact for the discrete map, branch by branch (limiter cases,
+HLLE upwind / subsonic branch, Davis speeds, floors, the periodic /
+zero-gradient halo fill).
 
 State tensors are float64 CUDA tensors in the problem-major interior layout:
 [4, nx, ny] for a mesh, [B, 4, nx, ny] for a batch of B equal-size problems

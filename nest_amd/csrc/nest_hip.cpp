@@ -20,11 +20,14 @@
 #include "nest_hip.h"
 #include "plan.hpp"
 
-// Buffer sets of the rollout adjoint's recomputed intervals: the recompute
-// of an interval (auxiliary stream) runs up to NEST_ADJ_SETS - 1 intervals
-// ahead of the reverse sweep.
+// Buffer sets of the rollout adjoint's checkpoint intervals: the rollout
+// keeps its last NEST_ADJ_SETS intervals in them (their reverse recomputes
+// nothing), and the recompute of an earlier interval (auxiliary stream) runs
+// up to NEST_ADJ_SETS - 1 intervals ahead of the reverse sweep. Each set
+// holds 2m - 1 states of an m-step interval. Four sets: at 1024^2, K = 200,
+// m = 15, the checkpointed gradient takes 1.2 ms (3%) less than with two.
 #ifndef NEST_ADJ_SETS
-#define NEST_ADJ_SETS 2
+#define NEST_ADJ_SETS 4
 #endif
 
 namespace {
@@ -182,7 +185,7 @@ struct nest_sim2d {
     std::vector<long long> goff_h;  // problem offsets in the problem-major interior layout
     long long gtotal = 0;
     long long* goff_d = nullptr;
-    std::vector<double*> ckpt;  // rollout checkpoints (every roll_m steps)
+    std::vector<double*> ckpt;  // rollout checkpoints (one per interval: its first state)
     // recomputed states of a checkpoint interval and their stage-1 states,
     // NEST_ADJ_SETS sets: the recompute of one interval (auxiliary stream)
     // overlaps the reverse sweep of the next (handle's stream)
@@ -191,7 +194,7 @@ struct nest_sim2d {
     hipEvent_t ev_start = nullptr;
     std::vector<hipEvent_t> ev_rec_c, ev_rev_c;  // per interval: recompute done, reverse done
     int roll_K = 0, roll_m = 0;
-    bool roll_kept = false;  // the last interval's states are in its buffer set
+    int roll_kept = 0;  // the last roll_kept intervals' states are in their buffer sets
     double roll_dt = 0.0;
 
     ~nest_sim2d() { release(); }
@@ -229,7 +232,7 @@ struct nest_sim2d {
         adj_U1 = adj_Vb = adj_Ub1 = nullptr;
         goff_d = nullptr;
         roll_K = 0;
-        roll_kept = false;
+        roll_kept = 0;
         U_own = U1_own = U = U1 = nullptr;
         patches_d = nullptr;
         gitems_d[0] = gitems_d[1] = nullptr;
@@ -1055,6 +1058,18 @@ int nest_hip_sim2d_rk2_vjp(nest_sim2d* s, const double* U0, double* Ubar, double
     return NEST_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Interval layout of a rollout of K steps with interval m: nck = ceil(K / m)
+// intervals; interval 0 takes the remainder (1..m steps) and every later one
+// m steps, so that the intervals the forward keeps (the last ones) are whole.
+// Interval c holds steps [interval_start(c), interval_start(c + 1)).
+int interval_start(int K, int m, int nck, int c) { return c <= 0 ? 0 : (c >= nck ? K : K - (nck - c) * m); }
+}  // namespace
+
+extern "C" {
+
 int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
     if (int rc = valid_sim(s)) return rc;
     if (nsteps < 1 || !(dt > 0.0)) return fail(NEST_ERR_INVALID, "nsteps >= 1 and dt > 0");
@@ -1064,39 +1079,37 @@ int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
     const int nck = (nsteps + m - 1) / m;
     if (int rc = s->grow(s->ckpt, (size_t)nck)) return rc;
     // With two or more intervals (or every state kept), the states of the
-    // last interval (and their stage-1 states) are written straight into the
-    // buffer set its reverse reads, so the reverse sweep starts without
-    // recomputing it. A single interval of a plain forward (interval >=
-    // nsteps) keeps nothing: its reverse recomputes it from the initial state.
-    const bool keep = all || nck >= 2;
+    // last min(NEST_ADJ_SETS, nck) intervals (and their stage-1 states) are
+    // written straight into the buffer sets their reverse reads, so the
+    // reverse sweep recomputes none of them. A single interval of a plain
+    // forward (interval >= nsteps) keeps nothing: its reverse recomputes it
+    // from the initial state.
     const int nsets = std::min(NEST_ADJ_SETS, nck);
-    const int cl = nck - 1, setl = cl % nsets, lenl = nsteps - cl * m;
-    if (keep) {
-        if (int rc = s->grow(s->seg[setl], (size_t)std::max(0, m - 1))) return rc;
-        if (int rc = s->grow(s->seg1[setl], (size_t)m)) return rc;
+    const int nkeep = all ? 1 : (nck >= 2 ? nsets : 0);
+    for (int c = nck - nkeep; c < nck; ++c) {
+        if (int rc = s->grow(s->seg[c % nsets], (size_t)std::max(0, m - 1))) return rc;
+        if (int rc = s->grow(s->seg1[c % nsets], (size_t)m)) return rc;
     }
-    // checkpoint c = state of step c*m: the initial state is copied; every
-    // later one is written by the stage 2 before it (out of place) and read
-    // from there by the next step, whose stage 2 writes U again
+    // checkpoint c = first state of interval c: the initial state is copied;
+    // every later one is written by the stage 2 before it (out of place) and
+    // read from there by the next step
     NEST_HIP_TRY(hipMemcpyAsync(s->ckpt[0], s->U, s->total * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-    for (int k = 0; k < nsteps; ++k) {
-        if (keep && k >= cl * m) {
-            // last interval: state i in the set (the checkpoint first), the
-            // final state into U
-            const int i = k - cl * m;
-            double* src = i == 0 ? s->ckpt[cl] : s->seg[setl][i - 1];
-            double* dst = i + 1 < lenl ? s->seg[setl][i] : s->U;
-            s->stage_on(1, src, s->seg1[setl][i], dt);
-            s->stage_on(2, src, s->seg1[setl][i], dt, dst);
-            continue;
+    for (int c = 0, k = 0; c < nck; ++c) {
+        const int len = interval_start(nsteps, m, nck, c + 1) - interval_start(nsteps, m, nck, c);
+        const bool kept = c >= nck - nkeep;
+        const int set = c % nsets;
+        for (int i = 0; i < len; ++i, ++k) {
+            // the state after this step: the next checkpoint, the final
+            // state (U), or (kept interval) state i + 1 in the set
+            double* next = i + 1 < len ? (kept ? s->seg[set][i] : s->U) : (c + 1 < nck ? s->ckpt[c + 1] : s->U);
+            double* src = i == 0 ? (k > 0 ? s->ckpt[c] : s->U) : (kept ? s->seg[set][i - 1] : s->U);
+            double* u1 = kept ? s->seg1[set][i] : s->U1;
+            s->stage_on(1, src, u1, dt);
+            s->stage_on(2, src, u1, dt, next == src ? nullptr : next);
         }
-        double* src = (k % m == 0 && k > 0) ? s->ckpt[k / m] : s->U;
-        double* dst = ((k + 1) % m == 0 && k + 1 < nsteps) ? s->ckpt[(k + 1) / m] : s->U;
-        s->stage_on(1, src, s->U1, dt);
-        s->stage_on(2, src, s->U1, dt, dst == src ? nullptr : dst);
     }
     s->fill(s->U);
-    s->roll_kept = keep;
+    s->roll_kept = nkeep;
     s->roll_K = nsteps;
     s->roll_m = m;
     s->roll_dt = dt;
@@ -1123,16 +1136,20 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     // one event per interval and direction (an event is never recorded
     // again while a wait on its previous record may still be pending)
     if (int rc = s->ensure_interval_events(nck)) return rc;
-    // states of steps c*m .. c*m + len - 1 of interval c: the checkpoint, then
-    // recomputed into buffer set c % S, with the stage-1 state of each
+    // states of interval c: the checkpoint, then the recomputed (or kept)
+    // states in buffer set c % S, with the stage-1 state of each
     auto state = [&](int c, int i) { return i == 0 ? s->ckpt[c] : s->seg[c % nsets][i - 1]; };
+    auto length = [&](int c) { return interval_start(K, m, nck, c + 1) - interval_start(K, m, nck, c); };
+    // the last roll_kept intervals were kept by the rollout
+    const int nkept = s->roll_kept;
+    auto kept = [&](int c) { return c >= nck - nkept; };
     auto recompute = [&](int c) -> hipError_t {
         // interval c reuses the set that the reverse of interval c + S read
         if (c + nsets <= nck - 1) {
             hipError_t e = hipStreamWaitEvent(s->aux_stream, s->ev_rev_c[c + nsets], 0);
             if (e != hipSuccess) return e;
         }
-        const int len = std::min(m, K - c * m), set = c % nsets;
+        const int len = length(c), set = c % nsets;
         for (int i = 0; i < len; ++i) {
             s->stage_on(1, state(c, i), s->seg1[set][i], dt, nullptr, s->aux_stream);
             if (i + 1 < len) s->stage_on(2, state(c, i), s->seg1[set][i], dt, state(c, i + 1), s->aux_stream);
@@ -1142,29 +1159,29 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     // everything enqueued before (the rollout and its checkpoints) comes first
     NEST_HIP_TRY(hipEventRecord(s->ev_start, s->stream));
     NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_start, 0));
-    // the last interval's states were kept by the rollout (with 2+ intervals,
-    // or every state)
-    const bool kept = s->roll_kept;
     for (int c = nck - 1; c >= std::max(0, nck - (nsets - 1)); --c)
-        if (!(kept && c == nck - 1)) NEST_HIP_TRY(recompute(c));
+        if (!kept(c)) NEST_HIP_TRY(recompute(c));
     // the recompute S - 1 intervals further down (auxiliary stream; it waits
     // for the reverse of interval c + 1, the last to read its buffer set),
     // then the reverse of interval c (handle's stream). The recompute goes
     // into the queue first, so the auxiliary stream has it in hand when the
     // reverse of c starts, however far ahead of the device the host is.
+    bool recomputed = false;
     for (int c = nck - 1; c >= 0; --c) {
-        const int set = c % nsets, len = std::min(m, K - c * m);
+        const int set = c % nsets, len = length(c);
         const int r = c - (nsets - 1);
-        if (r >= 0 && !(kept && r == nck - 1)) NEST_HIP_TRY(recompute(r));
-        if (!(kept && c == nck - 1)) NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
+        if (r >= 0 && !kept(r)) NEST_HIP_TRY(recompute(r));
+        if (!kept(c)) {
+            NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
+            recomputed = true;
+        }
         for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
         NEST_HIP_TRY(hipEventRecord(s->ev_rev_c[c], s->stream));
     }
-    // With more intervals than buffer sets, the recompute of interval
-    // nck - 1 - nsets (or an earlier one of the same set) has overwritten the
-    // kept last interval: a further reverse sweep of this rollout (a second
-    // backward of the same forward) must recompute it.
-    if (nck > nsets) s->roll_kept = false;
+    // A recompute overwrites the buffer set of a kept interval: a further
+    // reverse sweep of this rollout (a second backward of the same forward)
+    // must then recompute every interval.
+    if (recomputed) s->roll_kept = 0;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }

@@ -260,14 +260,16 @@ def test_adjoint_rejects_reflecting_walls():
     assert e.value.code == 5 and "reflecting" in str(e.value)
 
 
-@pytest.mark.parametrize("interval", [2, 3, 4, -1])
+@pytest.mark.parametrize("interval", [1, 2, 3, -1])
 def test_rollout_adjoint_twice(oracle, interval):
     """Two reverse sweeps of one rollout (as a second backward of the same
     forward makes) give bitwise the same cotangent. K = 11 with checkpoints
-    every 2, 3 or 4 steps is 6, 4 or 3 intervals, more than the reverse
-    sweep's two buffer sets: the first sweep's recomputes reuse the buffer
-    set the rollout kept its last interval in, so the second sweep must
-    recompute that interval. Both equal the dual-number dot-product check."""
+    every 1 or 2 steps is 11 or 6 intervals, more than the reverse sweep's
+    four buffer sets: the first sweep's recomputes reuse the buffer sets the
+    rollout kept its last four intervals in, so the second sweep must
+    recompute every interval. Every 3 steps (4 intervals, all kept) and every
+    state kept recompute nothing, in either sweep. Both equal the
+    dual-number dot-product check."""
     nx, ny, npx, npy, K = 64, 48, 2, 2, 11
     print_case("adjoint rollout twice", interval)
     m = MeshSpec(nx, ny, npx, npy, bcx="periodic", bcy="periodic")
