@@ -366,7 +366,9 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
     `steps` RK2 steps with device dt (graph-replayed); then `nadj`
     concurrent adjoint gradients of 256^2 instances of the adjoint leg's setup
     (each its own seed) in one batch handle, K = adj_steps, against the same
-    gradients one problem at a time."""
+    gradients one problem at a time; then both together (the KH steps and the
+    batched gradients enqueued at once on their handles' own streams, which
+    run side by side), against the two run alone."""
     import numpy as np
     import torch
 
@@ -375,17 +377,22 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
 
     n, npch = 8192, 64
     sim = Sim2D.mesh(n, n, npch, npch, bcx="periodic", bcy="periodic", mode=mode)
-    sim.upload(sim.scatter(P.kelvin_helmholtz(n)))
-    sim.fill_halos()
-    sim.start(cfl=0.4)
+    U_kh = sim.scatter(P.kelvin_helmholtz(n))
+
+    def kh_reset():
+        sim.upload(U_kh)
+        sim.fill_halos()
+        sim.start(cfl=0.4)
+        sim.synchronize()
+
+    kh_reset()
     sim.advance(2, graph=True)
-    sim.synchronize()
+    kh_reset()
     t0 = time.perf_counter()
     sim.advance(steps, graph=True)
     sim.synchronize()
     t_kh = time.perf_counter() - t0
     st = sim.status()
-    sim.close()
     kh = {"cells": n * n, "patches": npch * npch, "steps": steps,
           "mcups": round(n * n * 2.0 * steps / t_kh / 1e6, 1), "ms_per_step": round(t_kh * 1e3 / steps, 3),
           "t_reached": float(f"{st['t'][0]:.6g}")}
@@ -422,6 +429,16 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
     t_b = timed(batched)
     t_f = timed(forward)
     kept = rb.rollout_interval() < 0
+    kh_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(steps, graph=True)  # enqueued on the KH handle's stream
+    batched()  # the rollout's own stream
+    torch.cuda.synchronize()
+    sim.synchronize()
+    t_both = time.perf_counter() - t0
+    st_both = sim.status()
+    sim.close()
     rb.close()
     rs = Rollout.mesh(m, m, nsteps=adj_steps, dt=dt, mode=mode)
     rs.tune([(m, m)], log=log)
@@ -441,7 +458,11 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
         "adjoint_fwd_cost": round(t_b / t_f, 3),
         "gradients_per_s": round(nadj / t_b, 2),
         "single_problem_gradient_ms": round(t_s * 1e3, 2),
-        "batching_speedup": round(t_s * nadj / t_b, 2)}}
+        "batching_speedup": round(t_s * nadj / t_b, 2)},
+        "together": {"ms": round(t_both * 1e3, 2), "kh_alone_ms": round(t_kh * 1e3, 2),
+                     "adjoints_alone_ms": round(t_b * 1e3, 2),
+                     "sum_over_together": round((t_kh + t_b) / t_both, 3),
+                     "kh_steps": int(st_both["steps"][0])}}
 
 
 def load_traffic(name: str):
