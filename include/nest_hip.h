@@ -241,13 +241,15 @@ int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt);
 /* Device-resident time stepping. start(): t = 0, dt policy per problem
    (fixed_dt > 0 holds dt fixed, else CFL with cfl), stop at t_end (use
    INFINITY for a fixed step count), and reduces the initial wave speeds.
-   advance(): nsteps steps of [dt finalize, one-launch step] (the state
-   alternating between U and U1; after an odd count it is copied back to U)
-   or of [dt finalize, stage 1, stage 2] (set_step_kernel), halos gathered
-   from neighbours as in rk2_step, enqueued without any host
-   synchronisation, then one halo exchange of U; use_graph != 0 replays a
-   captured graph of two steps (one-launch step) or one step. status(): per-problem t, dt, steps and done flags
-   (arrays of nprob; any may be NULL), blocking. */
+   advance(): nsteps steps of the one-launch step, which computes each
+   step's dt in the kernel from the wave speeds the previous step left (the
+   state alternating between U and U1; after an odd count it is copied back
+   to U), or of [dt finalize, stage 1, stage 2] (set_step_kernel), halos
+   gathered from neighbours as in rk2_step, enqueued without any host
+   synchronisation, then one halo exchange of U; use_graph != 0 replays
+   captured graphs of four steps (the last nsteps % 4 steps run outside
+   them). A done problem passes its state through. status(): per-problem t,
+   dt, steps and done flags (arrays of nprob; any may be NULL), blocking. */
 int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_dt);
 int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph);
 int nest_hip_sim2d_status(nest_sim2d* s, double* t, double* dt, int* steps, int* done);
@@ -305,7 +307,7 @@ int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval,
 /* advance() without a graph, with HIP events recorded on the handle's stream
    around each launch of the step; blocks at the end. ms[3] receives the
    summed durations of [dt finalize, stage 1, stage 2] over the nsteps steps
-   (one-launch step: [dt finalize, step kernel, 0]), *total_ms the elapsed
+   (one-launch step: [0, step kernel, 0]), *total_ms the elapsed
    time of the whole sequence. */
 int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* total_ms);
 

@@ -684,6 +684,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // its state through, so the two buffers stay interchangeable.
 // LDS: 8 rows x 4 variables x 64 doubles = 16 KB per wave.
 // ---------------------------------------------------------------------------
+// dt of the next step of one problem at time t from the max wave speeds
+// (bit patterns of |u|+c, |v|+c) of its newest state (CFL or fixed, clamped
+// to t_end): returns 0 (step), 1 (t_end reached) or 2 (non-finite or
+// non-positive dt, left in dt).
+__device__ inline int step_dt(const nest_prob_state& s, double t, unsigned long long bx, unsigned long long by,
+                              double& dt) {
+    if (!(t < s.t_end)) return 1;
+    const double sx = __longlong_as_double((long long)bx);
+    const double sy = __longlong_as_double((long long)by);
+    if (s.fixed_dt > 0.0) {
+        dt = s.fixed_dt;
+    } else if (s.dim == 1) {
+        dt = s.cfl * s.dx / sx;
+    } else {
+        double dtx = 1e10, dty = 1e10;
+        if (sx > 0.0) dtx = dmin(dtx, s.dx / sx);
+        if (sy > 0.0) dty = dmin(dty, s.dy / sy);
+        dt = s.cfl * dmin(dtx, dty);
+    }
+    dt = dmin(dt, s.t_end - t);
+    if (!(dt > 0.0) || dt != dt || dt == __builtin_inf() || sx != sx || sy != sy) return 2;
+    return 0;
+}
 constexpr int kStepRing = 8;
 constexpr int kWaveLdsStep = kStepRing * kRowSlot;
 // 1: the scheduler keeps stage 1 and stage 2 of an iteration apart
@@ -725,7 +748,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
         }
         const bool store = lane >= 4 && lane < 60 && g < G.NY;
         const int r0 = w.r0, r1 = w.r1;
-        if (!(a.host_dt > 0.0) && ps->done) {
+        // The step's dt: the caller's, or (device dt) computed by every work
+        // item from the speed slots the previous step wrote, with
+        // finalize_kernel's arithmetic (-1: the problem was already done).
+        // The problem's first work item records the step (dt, t, step count
+        // or done flag) and clears the slot set the next step writes. No
+        // wave reads what another wave of the launch writes: the time comes
+        // from tn[par], the record goes to tn[par ^ 1], and a done flag set
+        // here gives every item the same verdict (pass the state through).
+        double dt = a.host_dt;
+        bool pass = false;
+        if (!(a.host_dt > 0.0)) {
+            const unsigned long long* sr = a.slots_rd + ((size_t)pd.prob * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
+            const unsigned long long bx = wave_max_u64(sr[0]);
+            const unsigned long long by = wave_max_u64(sr[1]);
+            const double ctin = ps->tn[a.par];
+            double cdt = 0.0;
+            const int cdone = ps->done ? -1 : step_dt(*ps, ctin, bx, by, cdt);
+            if (it == a.first_item[pd.prob]) {
+                unsigned long long* sc = a.slots_clr + ((size_t)pd.prob * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
+                sc[0] = 0ull;
+                sc[1] = 0ull;
+                if (lane == 0) {
+                    if (cdone == 0) {
+                        ps->dt = cdt;
+                        ps->t = ctin + cdt;
+                        ps->tn[a.par ^ 1] = ctin + cdt;
+                        ps->steps += 1;
+                    } else if (cdone > 0) {
+                        ps->done = cdone;
+                        if (cdone == 2) ps->dt = cdt;
+                    }
+                }
+            }
+            dt = cdt;
+            pass = cdone != 0;
+        }
+        if (pass) {
             // finished problem: its state passes through to the output buffer
             if (store) {
                 for (int i = r0; i < r1; ++i) {
@@ -739,7 +798,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
             }
             continue;
         }
-        const double dt = a.host_dt > 0.0 ? a.host_dt : ps->dt;
         const double dtdx = dt / ps->dx;
         const double dtdy = dt / ps->dy;
 
@@ -1077,8 +1135,12 @@ __global__ __launch_bounds__(256) void speeds2d_kernel(stage_args a) {
         smx = wave_max_u64(smx);
         smy = wave_max_u64(smy);
         if (lane == 0) {
-            atomicMax(&ps->smax[0], smx);
-            atomicMax(&ps->smax[1], smy);
+            unsigned long long* sl =
+                a.slots_rd ? a.slots_rd + ((size_t)pd.prob * NEST_SPEED_SLOTS + (it & (NEST_SPEED_SLOTS - 1))) *
+                                              NEST_SLOT_STRIDE
+                           : &ps->smax[0];
+            atomicMax(sl, smx);
+            atomicMax(sl + 1, smy);
         }
     }
 }
@@ -1109,60 +1171,54 @@ __global__ __launch_bounds__(256) void fill_regions_kernel(double* base, const n
 //    is monotone), init 1e10, cells with s == 0 skipped;
 // 1D compute_dt (sod1d.cpp:127): (cfl*dx)/max(|u|+c);
 // then dt = min(dt, t_end - t) (sod1d.cpp:222, sod2d.cpp:320).
-// Schedule the next step of one problem from the max wave speeds (bit
-// patterns of |u|+c, |v|+c) of its newest state: clear them, then set dt
-// (CFL, fixed, clamped to t_end), advance t and the step count, or flag the
-// problem done (t_end reached: 1; non-finite dt: 2).
-__device__ inline void schedule_step(nest_prob_state& s, unsigned long long bx, unsigned long long by) {
+// Schedule the next step (parity par) of one problem from the max wave
+// speeds of its newest state: clear smax, then set dt, advance t (and
+// tn[par ^ 1]) and the step count, or flag the problem done (t_end reached:
+// 1; non-finite dt: 2).
+__device__ inline void schedule_step(nest_prob_state& s, unsigned long long bx, unsigned long long by, int par) {
     s.smax[0] = 0ull;
     s.smax[1] = 0ull;
     if (s.done) return;
-    if (!(s.t < s.t_end)) {
-        s.done = 1;
-        return;
-    }
-    const double sx = __longlong_as_double((long long)bx);
-    const double sy = __longlong_as_double((long long)by);
-    double dt;
-    if (s.fixed_dt > 0.0) {
-        dt = s.fixed_dt;
-    } else if (s.dim == 1) {
-        dt = s.cfl * s.dx / sx;
-    } else {
-        double dtx = 1e10, dty = 1e10;
-        if (sx > 0.0) dtx = dmin(dtx, s.dx / sx);
-        if (sy > 0.0) dty = dmin(dty, s.dy / sy);
-        dt = s.cfl * dmin(dtx, dty);
-    }
-    dt = dmin(dt, s.t_end - s.t);
-    if (!(dt > 0.0) || dt != dt || dt == __builtin_inf() || sx != sx || sy != sy) {
-        s.done = 2;
-        s.dt = dt;
+    double dt = 0.0;
+    const int d = step_dt(s, s.t, bx, by, dt);
+    if (d) {
+        s.done = d;
+        if (d == 2) s.dt = dt;
         return;
     }
     s.dt = dt;
     s.t = s.t + dt;
+    s.tn[par ^ 1] = s.t;
     s.steps += 1;
 }
 
-__global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
+// Step scheduling of the two-launch step: folds and clears the slot set the
+// previous step wrote (slots), clears the set the next step writes
+// (slots_clr, may be null) and schedules step parity par.
+__global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsigned long long* slots,
+                                                      unsigned long long* slots_clr, int nprob, int par) {
     const int i = blockIdx.x;  // one wave per problem
     const int lane = threadIdx.x;
     if (i >= nprob) return;
     unsigned long long vx = 0, vy = 0;
-    if (slots) {  // fold and clear the stage-2 slots (lane k: slot k)
+    if (slots) {  // lane k: slot k
         unsigned long long* sl = slots + ((size_t)i * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
         vx = sl[0];
         vy = sl[1];
         sl[0] = 0ull;
         sl[1] = 0ull;
     }
+    if (slots_clr) {
+        unsigned long long* sc = slots_clr + ((size_t)i * NEST_SPEED_SLOTS + lane) * NEST_SLOT_STRIDE;
+        sc[0] = 0ull;
+        sc[1] = 0ull;
+    }
     vx = wave_max_u64(vx);
     vy = wave_max_u64(vy);
     if (lane != 0) return;
     nest_prob_state& s = P[i];
-    // speeds2d / the 1D kernels reduce straight into smax
-    schedule_step(s, s.smax[0] > vx ? s.smax[0] : vx, s.smax[1] > vy ? s.smax[1] : vy);
+    // the blocking compute_dt paths reduce straight into smax
+    schedule_step(s, s.smax[0] > vx ? s.smax[0] : vx, s.smax[1] > vy ? s.smax[1] : vy, par);
 }
 
 __global__ __launch_bounds__(64) void reset_speeds_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {
@@ -1687,9 +1743,10 @@ void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars
     hipLaunchKernelGGL(fill_regions_kernel, nreg, 256, 0, s, base, R, nreg, nvars);
 }
 
-void launch_finalize(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s) {
+void launch_finalize(nest_prob_state* P, unsigned long long* slots, unsigned long long* slots_clr, int nprob,
+                     int par, hipStream_t s) {
     if (nprob <= 0) return;
-    hipLaunchKernelGGL(finalize_kernel, nprob, NEST_SPEED_SLOTS, 0, s, P, slots, nprob);
+    hipLaunchKernelGGL(finalize_kernel, nprob, NEST_SPEED_SLOTS, 0, s, P, slots, slots_clr, nprob, par);
 }
 
 void launch_reset_speeds(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s) {

@@ -14,7 +14,8 @@
     int step2d_resident_blocks(int lim, int block);                                                 \
     void launch_speeds2d(const stage_args& a, int grid, int block, hipStream_t s);                  \
     void launch_fill_regions(double* base, const nest_region* R, int nreg, int nvars, hipStream_t s); \
-    void launch_finalize(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s);  \
+    void launch_finalize(nest_prob_state* P, unsigned long long* slots, unsigned long long* slots_clr,  \
+                         int nprob, int par, hipStream_t s);                                          \
     void launch_reset_speeds(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s); \
     void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s);                  \
     void launch_step1d(int lim, const stage1d_args& a, hipStream_t s);                              \
@@ -47,6 +48,15 @@ struct stage_args {
     long long total;  // doubles in each state buffer (all share the patch layout)
     unsigned long long* slots;  // stage 2: per-problem wave-speed slots (NEST_SPEED_SLOTS each)
     int kernel;  // stage kernel: 0 automatic, 1 register path, 2 LDS-ring path (state below 4 GiB)
+    // one-launch step with the dt computed in the kernel (device dt): the
+    // slot set the previous step wrote (read), the set the next step writes
+    // (cleared by each problem's first work item), each problem's first work
+    // item, the step's time parity. speeds2d: slots_rd, if set, receives the
+    // speeds instead of the problems' smax.
+    unsigned long long* slots_rd;
+    unsigned long long* slots_clr;
+    const int* first_item;
+    int par;
 };
 
 // arguments of the two adjoint kernels of one split stage S = Y o X (the

@@ -52,7 +52,7 @@ struct launch_table {
     int (*step2d_blocks)(int, int);
     void (*speeds2d)(const stage_args&, int, int, hipStream_t);
     void (*fill_regions)(double*, const nest_region*, int, int, hipStream_t);
-    void (*finalize)(nest_prob_state*, unsigned long long*, int, hipStream_t);
+    void (*finalize)(nest_prob_state*, unsigned long long*, unsigned long long*, int, int, hipStream_t);
     void (*reset_speeds)(nest_prob_state*, unsigned long long*, int, hipStream_t);
     void (*stage1d)(int, int, const stage1d_args&, hipStream_t);
     void (*step1d)(int, const stage1d_args&, hipStream_t);
@@ -150,16 +150,27 @@ struct nest_sim2d {
     nest_region* regions_d = nullptr;
     nest_item* items_d = nullptr;
     nest_prob_state* probs_d = nullptr;
-    unsigned long long* slots_d = nullptr;  // stage-2 wave-speed slots, NEST_SPEED_SLOTS per problem
+    // wave-speed slots of the device-dt steps: kSlotSets rotating sets of
+    // NEST_SPEED_SLOTS per problem. Device step n reads set n % 4 (written by
+    // step n - 1), writes set (n + 1) % 4 and clears set (n + 2) % 4 for
+    // step n + 1. Four sets, so that a graph of four steps starts and ends
+    // in the same phase.
+    static constexpr int kSlotSets = 4;
+    unsigned long long* slots_d = nullptr;
+    int nstep_dev = 0;  // device-dt steps since start (the rotation phase and time parity)
     nest_item* gitems_d[2] = {nullptr, nullptr};
     nest_item* fitems_d = nullptr;
+    int* first_item_d = nullptr;  // per problem: its first work item in fitems
     nest_prob_geom* geoms_d = nullptr;
     int* itab_d = nullptr;
 
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    // captured device-dt step graphs of kGraphSteps steps, one per rotation
+    // phase they start in
+    static constexpr int kGraphSteps = 4;
+    hipGraph_t graph[kSlotSets] = {};
+    hipGraphExec_t graph_exec[kSlotSets] = {};
 
     int rows = 64;              // per-patch strips
     int wpb = 4;
@@ -200,16 +211,18 @@ struct nest_sim2d {
     ~nest_sim2d() { release(); }
 
     void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+        for (int k = 0; k < kSlotSets; ++k) {
+            if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            graph_exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
     }
     void release() {
         drop_graph();
         for (void* p : {(void*)U_own, (void*)U1_own, (void*)patches_d, (void*)regions_d, (void*)items_d,
                         (void*)probs_d, (void*)slots_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)fitems_d,
-                        (void*)geoms_d, (void*)itab_d,
+                        (void*)first_item_d, (void*)geoms_d, (void*)itab_d,
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
                         (void*)goff_d})
             if (p) (void)hipFree(p);
@@ -237,6 +250,7 @@ struct nest_sim2d {
         patches_d = nullptr;
         gitems_d[0] = gitems_d[1] = nullptr;
         fitems_d = nullptr;
+        first_item_d = nullptr;
         geoms_d = nullptr;
         itab_d = nullptr;
         regions_d = nullptr;
@@ -248,6 +262,9 @@ struct nest_sim2d {
     }
 
     const launch_table& L() const { return table(mode); }
+    unsigned long long* slot_set(int k) const {
+        return slots_d + (size_t)(k & (kSlotSets - 1)) * probs_h.size() * NEST_SPEED_SLOTS * NEST_SLOT_STRIDE;
+    }
 
     // gather = 1 or 2: that stage's global strips (gitems) reading every cell,
     // halo included, from the patch that owns it (no exchange needed before
@@ -260,7 +277,7 @@ struct nest_sim2d {
         a.items = (gather ? gitems_d[gather - 1] : items_d) + item0;
         a.nitems = nitems;
         a.probs = probs_d;
-        a.slots = slots_d;
+        a.slots = slot_set(nstep_dev);  // host-dt stage 2: into the next device step's read set
         a.eos = eos;
         a.host_dt = host_dt;
         a.gather = gather;
@@ -304,8 +321,10 @@ struct nest_sim2d {
             }
         }
         fitems_h.clear();
+        std::vector<int> first_item_h(tb.geoms.size(), 0);
         for (int q = 0; q < (int)tb.geoms.size(); ++q) {
             const int r = (int)prob_rows[2].size() == nprob && prob_rows[2][q] > 0 ? prob_rows[2][q] : frows;
+            first_item_h[q] = (int)fitems_h.size();
             nest_plan::add_gather_items(fitems_h, tb, q, r, 56);
         }
         for (int k = 0; k < 2; ++k) {
@@ -330,6 +349,11 @@ struct nest_sim2d {
         }
         if (int rc = dev_alloc(&fitems_d, fitems_h.size())) return rc;
         NEST_HIP_TRY(hipMemcpy(fitems_d, fitems_h.data(), fitems_h.size() * sizeof(nest_item), hipMemcpyHostToDevice));
+        if (first_item_d) (void)hipFree(first_item_d);
+        first_item_d = nullptr;
+        if (int rc = dev_alloc(&first_item_d, first_item_h.size())) return rc;
+        NEST_HIP_TRY(hipMemcpy(first_item_d, first_item_h.data(), first_item_h.size() * sizeof(int),
+                               hipMemcpyHostToDevice));
         fresident = L().step2d_blocks(lim, 64 * fwpb) * fwpb * ncu;
         drop_graph();
         return NEST_OK;
@@ -359,7 +383,7 @@ struct nest_sim2d {
         if (int rc = dev_alloc(&patches_d, patches_h.size())) return rc;
         if (int rc = dev_alloc(&regions_d, regions_h.size())) return rc;
         if (int rc = dev_alloc(&probs_d, probs_h.size())) return rc;
-        const size_t nslot = probs_h.size() * (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE;
+        const size_t nslot = kSlotSets * probs_h.size() * (size_t)NEST_SPEED_SLOTS * NEST_SLOT_STRIDE;
         if (int rc = dev_alloc(&slots_d, nslot)) return rc;
         NEST_HIP_TRY(hipMemset(slots_d, 0, nslot * sizeof(unsigned long long)));
         if (int rc = dev_alloc(&geoms_d, tb.geoms.size())) return rc;
@@ -399,17 +423,22 @@ struct nest_sim2d {
         if (nitems <= 0) return;
         L().stage2d(st, lim, args(host_dt, item0, nitems, 0), grid_for(nitems, wpb), 64 * wpb, stream);
     }
-    // all global strips of stage st (halo gathered in the kernel)
-    void gstage(int st, double host_dt) {
+    // all global strips of stage st (halo gathered in the kernel); stage 2
+    // folds its speeds into `slots` (null: the default of args())
+    void gstage(int st, double host_dt, unsigned long long* slots = nullptr) {
         const int n = (int)gitems_h[st - 1].size();
         const int w = gwpb[st - 1];
-        L().stage2d(st, lim, args(host_dt, 0, n, st), grid_for(n, w), 64 * w, stream);
+        stage_args a = args(host_dt, 0, n, st);
+        if (slots) a.slots = slots;
+        L().stage2d(st, lim, a, grid_for(n, w), 64 * w, stream);
     }
     void speeds(int item0, int nitems) {
         if (nitems <= 0) return;
         L().speeds2d(args(0.0, item0, nitems), grid_for(nitems, wpb), 64 * wpb, stream);
     }
-    // one-launch step from src into dst (all global strips)
+    // one-launch step from src into dst (all global strips): the caller's
+    // dt (host_dt > 0), or device step nstep_dev with its dt computed in the
+    // kernel from the rotating slot sets
     void fstep(double host_dt, const double* src, double* dst) {
         const int n = (int)fitems_h.size();
         stage_args a = args(host_dt, 0, n, 0);
@@ -418,21 +447,35 @@ struct nest_sim2d {
         a.U = const_cast<double*>(src);
         a.U1 = nullptr;
         a.Uo = dst;
+        if (!(host_dt > 0.0)) {
+            a.slots_rd = slot_set(nstep_dev);
+            a.slots = slot_set(nstep_dev + 1);
+            a.slots_clr = slot_set(nstep_dev + 2);
+            a.first_item = first_item_d;
+            a.par = nstep_dev & 1;
+        }
         L().step2d(lim, a, grid_for(n, fwpb, fresident), 64 * fwpb, stream);
     }
-    // Step k of a device-dt sequence: dt finalize, then the one-launch step
-    // from U into U1 (even k) or from U1 into U (odd k), or the two stages
-    // (stage 1 U -> U1, stage 2 U1, U -> U). Every kernel gathers its halo
-    // from the neighbours (no exchange launches). Ghost cells of U are left
-    // stale; callers refresh them with fill(U) when they hand U out.
+    // dt of the two-launch step: fold the read set, clear the set the step
+    // after next writes, schedule the step
+    void finalize() {
+        L().finalize(probs_d, slot_set(nstep_dev), slot_set(nstep_dev + 2), nprob, nstep_dev & 1, stream);
+    }
+    // Step k of a device-dt sequence (device step nstep_dev, which it
+    // advances): the one-launch step, dt included, from U into U1 (even k)
+    // or from U1 into U (odd k); or dt finalize and the two stages (stage 1
+    // U -> U1, stage 2 U1, U -> U). Every kernel gathers its halo from the
+    // neighbours (no exchange launches). Ghost cells of U are left stale;
+    // callers refresh them with fill(U) when they hand U out.
     void device_step(int k) {
-        L().finalize(probs_d, slots_d, nprob, stream);
         if (one_launch()) {
             fstep(0.0, (k & 1) ? U1 : U, (k & 1) ? U : U1);
         } else {
+            finalize();
             gstage(1, 0.0);
-            gstage(2, 0.0);
+            gstage(2, 0.0, slot_set(nstep_dev + 1));
         }
+        ++nstep_dev;
     }
     // after a device-dt sequence of nsteps one-launch steps the state is in
     // U1 when nsteps is odd: bring it back to U
@@ -821,7 +864,8 @@ int nest_hip_sim2d_exchange(nest_sim2d* s, int buffer) {
 int nest_hip_sim2d_compute_dt(nest_sim2d* s, double cfl, double* dt_host) {
     if (int rc = valid_sim(s)) return rc;
     if (!dt_host) return fail(NEST_ERR_INVALID, "dt_host is null");
-    s->L().reset_speeds(s->probs_d, s->slots_d, s->nprob, s->stream);
+    // into smax (the device-dt slot sets keep the pending step's speeds)
+    s->L().reset_speeds(s->probs_d, nullptr, s->nprob, s->stream);
     s->speeds(0, (int)s->items_h.size());
     NEST_HIP_TRY(hipGetLastError());
     NEST_HIP_TRY(hipMemcpyAsync(s->probs_h.data(), s->probs_d, s->probs_h.size() * sizeof(nest_prob_state),
@@ -868,12 +912,19 @@ int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_d
         p.steps = 0;
         p.done = 0;
         p.smax[0] = p.smax[1] = 0ull;
+        p.tn[0] = p.tn[1] = 0.0;
     }
     NEST_HIP_TRY(hipMemcpyAsync(s->probs_d, s->probs_h.data(), s->probs_h.size() * sizeof(nest_prob_state),
                                 hipMemcpyHostToDevice, s->stream));
-    // stage-2 slots may hold maxima of earlier host-dt stages
-    s->L().reset_speeds(s->probs_d, s->slots_d, s->nprob, s->stream);
-    s->speeds(0, (int)s->items_h.size());
+    // every slot set may hold maxima of earlier steps; the speeds of the
+    // state go to the read set of device step 0
+    s->nstep_dev = 0;
+    for (int k = 0; k < nest_sim2d::kSlotSets; ++k) s->L().reset_speeds(s->probs_d, s->slot_set(k), s->nprob, s->stream);
+    {
+        stage_args a = s->args(0.0, 0, (int)s->items_h.size());
+        a.slots_rd = s->slot_set(0);
+        s->L().speeds2d(a, s->grid_for((int)s->items_h.size(), s->wpb), 64 * s->wpb, s->stream);
+    }
     NEST_HIP_TRY(hipGetLastError());
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
     return NEST_OK;
@@ -885,17 +936,25 @@ int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph) {
     if (!use_graph) {
         for (int k = 0; k < nsteps; ++k) s->device_step(k);
     } else {
-        // the graph holds one step, or two one-launch steps (U -> U1 -> U);
-        // an odd last step runs outside it
-        const int per = s->one_launch() ? 2 : 1;
-        if (!s->graph_exec) {
+        // a graph holds four steps (an even count: one-launch steps end in U;
+        // the slot-set rotation and time parity return to the phase it
+        // started in), one graph per starting phase; the last nsteps % 4
+        // steps run outside it
+        const int per = nest_sim2d::kGraphSteps;
+        const int ph = s->nstep_dev & (nest_sim2d::kSlotSets - 1);
+        if (nsteps >= per && !s->graph_exec[ph]) {
+            const int n0 = s->nstep_dev;
             NEST_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < per; ++k) s->device_step(k);
-            hipError_t e = hipStreamEndCapture(s->stream, &s->graph);
+            s->nstep_dev = n0;  // captured, not run
+            hipError_t e = hipStreamEndCapture(s->stream, &s->graph[ph]);
             if (e != hipSuccess) return fail(NEST_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-            NEST_HIP_TRY(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+            NEST_HIP_TRY(hipGraphInstantiate(&s->graph_exec[ph], s->graph[ph], nullptr, nullptr, 0));
         }
-        for (int r = 0; r < nsteps / per; ++r) NEST_HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+        for (int r = 0; r < nsteps / per; ++r) {
+            NEST_HIP_TRY(hipGraphLaunch(s->graph_exec[ph], s->stream));
+            s->nstep_dev += per;
+        }
         for (int k = nsteps / per * per; k < nsteps; ++k) s->device_step(k);
     }
     if (int rc = s->settle(nsteps)) return rc;
@@ -927,7 +986,9 @@ int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl) {
     std::vector<double> dts(s->nprob);
     for (int k = 0; k < nsteps; ++k) {
         // Reduce: wave speeds patch by patch, blocking read-back, host dt
-        s->L().reset_speeds(s->probs_d, s->slots_d, s->nprob, s->stream);
+        // (the read set of the next device step is cleared: stage 2 below
+        // leaves the new state's speeds there)
+        s->L().reset_speeds(s->probs_d, s->slot_set(s->nstep_dev), s->nprob, s->stream);
         for (int p = 0; p < np; ++p) s->speeds(s->item_begin[p], s->item_end[p] - s->item_begin[p]);
         NEST_HIP_TRY(hipMemcpyAsync(s->probs_h.data(), s->probs_d, s->probs_h.size() * sizeof(nest_prob_state),
                                     hipMemcpyDeviceToHost, s->stream));
@@ -960,7 +1021,7 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
         for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(k), with events between
             hipEvent_t* e = &ev[4 * (size_t)k];
             (void)hipEventRecord(e[0], s->stream);
-            s->L().finalize(s->probs_d, s->slots_d, s->nprob, s->stream);
+            if (!one) s->finalize();
             (void)hipEventRecord(e[1], s->stream);
             if (one) {
                 s->fstep(0.0, (k & 1) ? s->U1 : s->U, (k & 1) ? s->U : s->U1);
@@ -968,9 +1029,10 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
             } else {
                 s->gstage(1, 0.0);
                 (void)hipEventRecord(e[2], s->stream);
-                s->gstage(2, 0.0);
+                s->gstage(2, 0.0, s->slot_set(s->nstep_dev + 1));
             }
             (void)hipEventRecord(e[3], s->stream);
+            ++s->nstep_dev;
         }
         if (int r2 = s->settle(nsteps)) rc = r2;
         hipError_t e = hipStreamSynchronize(s->stream);

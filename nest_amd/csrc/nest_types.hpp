@@ -56,6 +56,11 @@ struct nest_prob_state {
     int done;         // 1: t reached t_end; 2: non-finite dt (error)
     int dim;          // 1 or 2 (dt formula)
     int pad;
+    // t at the end of the last device-dt step, by step parity: step k reads
+    // tn[k & 1] and writes tn[(k + 1) & 1], so the one-launch step, whose
+    // every wave computes the step's dt, never reads what its own launch
+    // writes
+    double tn[2];
 };
 
 // Time-stepping state of the 1D solver's fused device-dt loop. Step k reads

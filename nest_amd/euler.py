@@ -301,7 +301,8 @@ class Sim2D:
 
     def advance_timed(self, nsteps: int):
         """nsteps device-dt steps with HIP events around each launch; returns
-        (per-component ms [finalize, stage 1, stage 2], total ms)."""
+        (per-component ms [finalize, stage 1, stage 2], total ms); with the
+        one-launch step [0, step kernel, 0]."""
         ms = np.zeros(3)
         tot = C.c_double()
         check(self.L.nest_hip_sim2d_advance_timed(self.h, nsteps, _dptr(ms), C.byref(tot)))

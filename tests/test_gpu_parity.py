@@ -361,6 +361,54 @@ def test_2d_device_dt_run(oracle, graph, kernel):
     assert np.array_equal(sim.gather(sim.download()), sim.gather(ref))
 
 
+def test_2d_device_dt_mixed_sequence(oracle):
+    """One device-dt run of an uneven 3-problem batch (exact numerics) that
+    switches between the one-launch step (dt computed in the step kernel)
+    and the two-launch step (dt finalize + two stages), plain launches and
+    graph replays starting in every phase of the speed-slot rotation
+    (3 + 2 + 5 + 6 steps, then 40 more in graphs), and runs past t_end: each
+    problem's states after 16 steps and at the end, its step count, done
+    flag and time equal the oracle's compute_dt_2d loop for that problem
+    alone. The problems reach t_end at different steps, so finished ones
+    pass their state through while the others go on."""
+    sizes = [(64, 64), (48, 80), (96, 40)]
+    t_end = 0.1
+    print_case("2d device dt mixed sequence", sizes)
+    sim = Sim2D.batch(sizes, mode="exact")
+    U = np.zeros(sim.total)
+    refs = []
+    for k, (nx, ny) in enumerate(sizes):
+        m = MeshSpec(nx, ny, 1, 1)
+        u = m.to_patches(P.sod2d(nx, ny, jitter=1e-3, seed=5 + k))
+        oracle.mesh_fill(u, m)
+        sim.patch_view(U, k)[:] = u.reshape(4, nx + 4, ny + 4)
+        refs.append((m, u))
+    sim.upload(U)
+    sim.start(cfl=0.4, t_end=t_end)
+    for kind, n, graph in (("auto", 3, False), ("two_stage", 2, True), ("two_stage", 5, True),
+                           ("auto", 6, True)):
+        sim.set_step_kernel(kind)
+        sim.advance(n, graph=graph)
+    mid = sim.download()
+    st_mid = sim.status()
+    sim.advance(40, graph=True)
+    got = sim.download()
+    st = sim.status()
+    for k, (m, u) in enumerate(refs):
+        ref16 = u.copy()
+        k16, _ = oracle.mesh_run(ref16, m, 16, 1.0 / m.nx, 1.0 / m.ny, cfl=0.4, t_end=t_end)
+        assert k16 == 16 and st_mid["steps"][k] == 16 and st_mid["done"][k] == 0, (k, k16, st_mid)
+        shape = (4, m.nx + 4, m.ny + 4)
+        assert np.array_equal(sim.patch_view(mid, k)[:, 2:-2, 2:-2], ref16.reshape(shape)[:, 2:-2, 2:-2]), k
+        ref = u.copy()
+        k_ref, dts = oracle.mesh_run(ref, m, 1000, 1.0 / m.nx, 1.0 / m.ny, cfl=0.4, t_end=t_end)
+        assert k_ref < 56 and st["steps"][k] == k_ref and st["done"][k] == 1, (k, k_ref, st)
+        assert st["t"][k] == np.cumsum(dts)[-1] or abs(st["t"][k] - t_end) < 1e-15
+        assert np.array_equal(sim.patch_view(got, k)[:, 2:-2, 2:-2], ref.reshape(shape)[:, 2:-2, 2:-2]), k
+    assert len(set(int(x) for x in st["steps"])) > 1  # the problems finished at different steps
+    sim.close()
+
+
 @pytest.mark.parametrize("kernel", STAGE_KERNELS)
 def test_2d_limiters(oracle, kernel):
     nx, ny = 48, 40
