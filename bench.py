@@ -562,6 +562,7 @@ def main():
     st = sim.status()
     assert st["done"][0] == 0 and st["steps"][0] == args.warmup + extra_warm + 2 + 3 * args.steps
     one = sim.one_launch
+    forced_tiling = sim.get_step_tiling() if (args.rows or args.wpb) else None
     k1_ms = comp_ms[1] / args.steps
     k2_ms = comp_ms[2] / args.steps
     if one:
@@ -648,9 +649,19 @@ def main():
 
     copies = copy_peak_gbs()
     copy_gbs = copies["library_copy_kernel"]
+    step_tiling = None
     if one:
+        # measured HBM bytes per launch of the tiling this run used (rows per
+        # work item x waves per block), from the rocprof counter passes
+        if forced_tiling:
+            step_tiling = forced_tiling
+        else:
+            from nest_amd import autotune
+            cls = autotune.size_classes([(N, N)])[0]
+            step_tiling = (tiling["3"]["rows"][cls], tiling["3"]["waves"])
         traffic = load_traffic("traffic_step2d_sod4096.json")
-        tr_bytes = (traffic or {}).get("step_bytes_per_launch")
+        entry = ((traffic or {}).get("tilings") or {}).get(f"{step_tiling[0]}x{step_tiling[1]}") or {}
+        tr_bytes = entry.get("step_bytes_per_launch")
     else:
         traffic = load_traffic("traffic_stage2d_sod4096.json")
         tr_bytes = (traffic or {}).get("stage2_bytes_per_launch")
@@ -695,6 +706,7 @@ def main():
                      "measured_copy_peak": round(copy_gbs, 1), "frac_vs_copy_peak": round(achieved / copy_gbs, 4),
                      "copy_kernels": copies,
                      "traffic": tr_bytes,
+                     "traffic_tiling": f"{step_tiling[0]} rows x {step_tiling[1]} waves" if step_tiling else None,
                      # measured bytes (rocprof counters) moved per second of the launch
                      "traffic_gbs": round(tr_bytes / (kern_ms * 1e-3) / 1e9, 1) if tr_bytes else None},
         "cpu_baseline": cpu,
