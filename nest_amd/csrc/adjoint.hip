@@ -42,10 +42,6 @@
 #ifndef NEST_ADJX_WAVES
 #define NEST_ADJX_WAVES 2
 #endif
-// X kernel interior: two march positions per trip (1) or one (0)
-#ifndef NEST_ADJX_PAIR
-#define NEST_ADJX_PAIR 1
-#endif
 
 namespace NEST_NS {
 
@@ -488,12 +484,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
         const int tz = w.r1 + 3 - (hi ? 1 : 0);  // its end
         int t = w.r0;
         for (; t < ta && t < te; ++t) slow(t);
-#if NEST_ADJX_PAIR
         for (; t + 1 < tz; t += 2) {
             fast(t);
             fast(t + 1);
         }
-#endif
         for (; t < tz; ++t) fast(t);
         for (; t < te; ++t) slow(t);
     }
