@@ -528,6 +528,31 @@ def test_sod4096_full_size_one_step(oracle):
     assert np.array_equal(got, U)
 
 
+def test_state_over_4gib_runs_the_register_path(oracle):
+    """A mesh whose state buffer is 4 GiB or more (11648^2 on 8x8 patches of
+    1456^2: 545.7 M doubles per state). The one-launch step and the LDS-ring
+    stage kernels address buffers with 32-bit byte offsets, so the handle
+    must run the two-launch step with the register-path stage kernels
+    (64-bit addressing). One device-dt step (exact numerics) equals the
+    oracle's step, dt included, bitwise."""
+    n, npt = 11648, 8
+    print_case("state over 4 GiB", n)
+    sim = Sim2D.mesh(n, n, npt, npt, mode="exact")
+    assert sim.total * 8 >= 2**32 and not sim.one_launch
+    m = MeshSpec(n, n, npt, npt)
+    U = sim.scatter(P.sod2d(n, n))
+    oracle.mesh_fill(U, m)
+    sim.upload(U)
+    sim.start(cfl=0.4)
+    sim.advance(1)
+    st = sim.status()
+    got = m.to_global(sim.download())
+    sim.close()
+    k, dts = oracle.mesh_run(U, m, 1, 1.0 / n, 1.0 / n, cfl=0.4)
+    assert k == 1 and st["steps"][0] == 1 and st["dt"][0] == dts[0], (st, dts)
+    assert np.array_equal(got, m.to_global(U))
+
+
 def test_kh8192_full_size_one_step(oracle):
     """The Kelvin-Helmholtz mesh at full size (8192^2, periodic, 64x64
     patches of 128^2): one RK2 step with the CFL dt, bitwise vs the oracle in
