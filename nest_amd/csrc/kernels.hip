@@ -1172,12 +1172,9 @@ __global__ __launch_bounds__(256) void fill_regions_kernel(double* base, const n
 // 1D compute_dt (sod1d.cpp:127): (cfl*dx)/max(|u|+c);
 // then dt = min(dt, t_end - t) (sod1d.cpp:222, sod2d.cpp:320).
 // Schedule the next step (parity par) of one problem from the max wave
-// speeds of its newest state: clear smax, then set dt, advance t (and
-// tn[par ^ 1]) and the step count, or flag the problem done (t_end reached:
-// 1; non-finite dt: 2).
+// speeds of its newest state: set dt, advance t (and tn[par ^ 1]) and the
+// step count, or flag the problem done (t_end reached: 1; non-finite dt: 2).
 __device__ inline void schedule_step(nest_prob_state& s, unsigned long long bx, unsigned long long by, int par) {
-    s.smax[0] = 0ull;
-    s.smax[1] = 0ull;
     if (s.done) return;
     double dt = 0.0;
     const int d = step_dt(s, s.t, bx, by, dt);
@@ -1216,9 +1213,7 @@ __global__ __launch_bounds__(64) void finalize_kernel(nest_prob_state* P, unsign
     vx = wave_max_u64(vx);
     vy = wave_max_u64(vy);
     if (lane != 0) return;
-    nest_prob_state& s = P[i];
-    // the blocking compute_dt paths reduce straight into smax
-    schedule_step(s, s.smax[0] > vx ? s.smax[0] : vx, s.smax[1] > vy ? s.smax[1] : vy, par);
+    schedule_step(P[i], vx, vy, par);
 }
 
 __global__ __launch_bounds__(64) void reset_speeds_kernel(nest_prob_state* P, unsigned long long* slots, int nprob) {

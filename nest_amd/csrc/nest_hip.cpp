@@ -197,9 +197,10 @@ struct nest_sim2d {
     long long gtotal = 0;
     long long* goff_d = nullptr;
     std::vector<double*> ckpt;  // rollout checkpoints (one per interval: its first state)
-    // recomputed states of a checkpoint interval and their stage-1 states,
-    // NEST_ADJ_SETS sets: the recompute of one interval (auxiliary stream)
-    // overlaps the reverse sweep of the next (handle's stream)
+    // states of a checkpoint interval (kept by the rollout or recomputed)
+    // and their stage-1 states, NEST_ADJ_SETS sets: the recompute of one
+    // interval (auxiliary stream) overlaps the reverse sweep of a later one
+    // (handle's stream)
     std::vector<double*> seg[NEST_ADJ_SETS], seg1[NEST_ADJ_SETS];
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_start = nullptr;
@@ -888,6 +889,9 @@ int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch) {
 int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt) {
     if (int rc = valid_sim(s)) return rc;
     if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
+    // the step leaves the new state's speeds in the next device-dt step's
+    // read set (cleared first, so they are exactly the new state's)
+    s->L().reset_speeds(s->probs_d, s->slot_set(s->nstep_dev), s->nprob, s->stream);
     if (s->one_launch()) {
         s->fstep(dt, s->U, s->U1);
         NEST_HIP_TRY(hipMemcpyAsync(s->U, s->U1, s->total * sizeof(double), hipMemcpyDeviceToDevice, s->stream));

@@ -51,7 +51,7 @@ struct nest_prob_state {
     double cfl;       // CFL number
     double dx, dy;    // cell sizes (dy unused in 1D)
     double fixed_dt;  // > 0: use this dt every step instead of the CFL dt
-    unsigned long long smax[2];  // max wave speed (|u|+c, |v|+c) of the newest state, as bit patterns
+    unsigned long long smax[2];  // max wave speeds (|u|+c, |v|+c) the blocking compute_dt paths read back, as bit patterns
     int steps;        // steps started
     int done;         // 1: t reached t_end; 2: non-finite dt (error)
     int dim;          // 1 or 2 (dt formula)

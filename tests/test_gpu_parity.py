@@ -410,6 +410,37 @@ def test_2d_device_dt_mixed_sequence(oracle):
 
 
 @pytest.mark.parametrize("kernel", STAGE_KERNELS)
+def test_2d_host_dt_steps_between_device_dt_steps(oracle, kernel):
+    """Device-dt steps, then an RK2 step with the caller's dt (compute_dt of
+    the current state), then device-dt steps again (exact numerics, graph
+    and plain launches): every dt and state equal the oracle's. The host-dt
+    step leaves exactly the new state's speeds for the next device step."""
+    nx, ny, npx, npy = 72, 56, 2, 2
+    print_case("2d host dt between device dt", kernel)
+    sim = Sim2D.mesh(nx, ny, npx, npy)
+    configure(sim, kernel)
+    m = MeshSpec(nx, ny, npx, npy)
+    U = sim.scatter(P.sod2d(nx, ny, jitter=1e-3, seed=11))
+    oracle.mesh_fill(U, m)
+    sim.upload(U)
+    sim.start(cfl=0.4)
+    ref = U.copy()
+    dx, dy = 1.0 / nx, 1.0 / ny
+    sim.advance(3)
+    oracle.mesh_run(ref, m, 3, dx, dy, cfl=0.4)
+    dt = sim.compute_dt(0.4)[0]
+    assert dt == oracle.mesh_dt(ref, m, dx, dy, 0.4)
+    sim.rk2_step(dt)
+    oracle.mesh_rk2_step(ref, m, dt, dx, dy)
+    sim.advance(5, graph=True)
+    _, dts = oracle.mesh_run(ref, m, 5, dx, dy, cfl=0.4)
+    st = sim.status()
+    assert st["steps"][0] == 8 and st["dt"][0] == dts[-1], (st, dts)
+    assert np.array_equal(sim.gather(sim.download()), sim.gather(ref))
+    sim.close()
+
+
+@pytest.mark.parametrize("kernel", STAGE_KERNELS)
 def test_2d_limiters(oracle, kernel):
     nx, ny = 48, 40
     for lim in ["mc", "none"]:
