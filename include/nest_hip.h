@@ -113,8 +113,10 @@ int nest_hip_solver1d_exchange(nest_solver1d* s, int buffer);
    sets the CFL number, the end time (t_end = INFINITY for none) and the
    initial wave speeds; advance() enqueues nsteps fused steps (the kernel of
    nest_hip_solver1d_run; steps past t_end are no-ops) and leaves the
-   state in U, halo-complete. status() reads t, the last dt and the step
-   count (blocking). */
+   state in U, halo-complete. Each step's dt comes from the state it steps:
+   after upload, stage or rk2_step the next advance reduces the speeds of
+   the current state again first. status() reads t, the last dt and the
+   step count (blocking). */
 int nest_hip_solver1d_start(nest_solver1d* s, double cfl, double t_end);
 int nest_hip_solver1d_advance(nest_solver1d* s, int nsteps);
 int nest_hip_solver1d_status(nest_solver1d* s, double* t, double* dt, int* steps, int* done);
@@ -248,8 +250,12 @@ int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt);
    gathered from neighbours as in rk2_step, enqueued without any host
    synchronisation, then one halo exchange of U; use_graph != 0 replays
    captured graphs of four steps (the last nsteps % 4 steps run outside
-   them). A done problem passes its state through. status(): per-problem t,
-   dt, steps and done flags (arrays of nprob; any may be NULL), blocking. */
+   them). A done problem passes its state through. Each step's dt comes
+   from the state it steps: after upload, upload_patch, bind, device_ptrs
+   (the caller may write through them), stage, pack into the patch layout
+   or rollout, the next advance reduces the speeds of the current state
+   again first. status(): per-problem t, dt, steps and done flags (arrays
+   of nprob; any may be NULL), blocking. */
 int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_dt);
 int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph);
 int nest_hip_sim2d_status(nest_sim2d* s, double* t, double* dt, int* steps, int* done);

@@ -158,6 +158,10 @@ struct nest_sim2d {
     static constexpr int kSlotSets = 4;
     unsigned long long* slots_d = nullptr;
     int nstep_dev = 0;  // device-dt steps since start (the rotation phase and time parity)
+    // the state changed outside a step since the speeds the next device-dt
+    // step reads were reduced (upload, bind, stages, rollouts, writes through
+    // device pointers): the next advance reduces them again first
+    bool speeds_stale = false;
     nest_item* gitems_d[2] = {nullptr, nullptr};
     nest_item* fitems_d = nullptr;
     int* first_item_d = nullptr;  // per problem: its first work item in fitems
@@ -436,6 +440,16 @@ struct nest_sim2d {
     void speeds(int item0, int nitems) {
         if (nitems <= 0) return;
         L().speeds2d(args(0.0, item0, nitems), grid_for(nitems, wpb), 64 * wpb, stream);
+    }
+    // the speeds of the current state into the read set of the next device
+    // step
+    void refresh_speeds() {
+        const int n = (int)items_h.size();
+        L().reset_speeds(probs_d, slot_set(nstep_dev), nprob, stream);
+        stage_args a = args(0.0, 0, n);
+        a.slots_rd = slot_set(nstep_dev);
+        L().speeds2d(a, grid_for(n, wpb), 64 * wpb, stream);
+        speeds_stale = false;
     }
     // one-launch step from src into dst (all global strips): the caller's
     // dt (host_dt > 0), or device step nstep_dev with its dt computed in the
@@ -723,11 +737,13 @@ int nest_hip_sim2d_bind(nest_sim2d* s, double* dev_U, double* dev_U1) {
     s->U = dev_U ? dev_U : s->U_own;
     s->U1 = dev_U1 ? dev_U1 : s->U1_own;
     s->drop_graph();
+    s->speeds_stale = true;
     return NEST_OK;
 }
 
 int nest_hip_sim2d_device_ptrs(nest_sim2d* s, double** dev_U, double** dev_U1) {
     if (int rc = valid_sim(s)) return rc;
+    s->speeds_stale = true;  // the caller may write the state through them
     if (dev_U) *dev_U = s->U;
     if (dev_U1) *dev_U1 = s->U1;
     return NEST_OK;
@@ -816,6 +832,7 @@ int nest_hip_sim2d_upload(nest_sim2d* s, const double* host) {
     if (!host) return fail(NEST_ERR_INVALID, "host is null");
     NEST_HIP_TRY(hipMemcpyAsync(s->U, host, s->total * sizeof(double), hipMemcpyHostToDevice, s->stream));
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->speeds_stale = true;
     return NEST_OK;
 }
 
@@ -835,6 +852,7 @@ int nest_hip_sim2d_upload_patch(nest_sim2d* s, int patch, const double* host) {
     const auto& d = s->patches_h[patch];
     NEST_HIP_TRY(hipMemcpyAsync(s->U + d.off, host, patch_elems(d) * sizeof(double), hipMemcpyHostToDevice, s->stream));
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->speeds_stale = true;
     return NEST_OK;
 }
 
@@ -882,6 +900,7 @@ int nest_hip_sim2d_stage(nest_sim2d* s, int stage, double dt, int patch) {
     if (patch >= (int)s->patches_h.size()) return fail(NEST_ERR_INVALID, "patch out of range");
     if (patch < 0) s->stage(stage, dt, 0, (int)s->items_h.size());
     else s->stage(stage, dt, s->item_begin[patch], s->item_end[patch] - s->item_begin[patch]);
+    s->speeds_stale = true;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -900,6 +919,7 @@ int nest_hip_sim2d_rk2_step(nest_sim2d* s, double dt) {
         s->gstage(2, dt);
     }
     s->fill(s->U);
+    s->speeds_stale = false;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -924,11 +944,7 @@ int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_d
     // state go to the read set of device step 0
     s->nstep_dev = 0;
     for (int k = 0; k < nest_sim2d::kSlotSets; ++k) s->L().reset_speeds(s->probs_d, s->slot_set(k), s->nprob, s->stream);
-    {
-        stage_args a = s->args(0.0, 0, (int)s->items_h.size());
-        a.slots_rd = s->slot_set(0);
-        s->L().speeds2d(a, s->grid_for((int)s->items_h.size(), s->wpb), 64 * s->wpb, s->stream);
-    }
+    s->refresh_speeds();
     NEST_HIP_TRY(hipGetLastError());
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
     return NEST_OK;
@@ -937,6 +953,7 @@ int nest_hip_sim2d_start(nest_sim2d* s, double cfl, double t_end, double fixed_d
 int nest_hip_sim2d_advance(nest_sim2d* s, int nsteps, int use_graph) {
     if (int rc = valid_sim(s)) return rc;
     if (nsteps < 0) return fail(NEST_ERR_INVALID, "nsteps < 0");
+    if (nsteps > 0 && s->speeds_stale) s->refresh_speeds();
     if (!use_graph) {
         for (int k = 0; k < nsteps; ++k) s->device_step(k);
     } else {
@@ -1009,6 +1026,7 @@ int nest_hip_sim2d_advance_per_patch(nest_sim2d* s, int nsteps, double cfl) {
         for (int p = 0; p < np; ++p)
             s->stage(2, dts[s->patches_h[p].prob], s->item_begin[p], s->item_end[p] - s->item_begin[p]);
         NEST_HIP_TRY(hipGetLastError());
+        s->speeds_stale = false;  // stage 2 left the new state's speeds in the cleared read set
     }
     return NEST_OK;
 }
@@ -1021,6 +1039,7 @@ int nest_hip_sim2d_advance_timed(nest_sim2d* s, int nsteps, double* ms, double* 
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) rc = fail(NEST_ERR_HIP, "hipEventCreate failed");
     if (!rc) {
+        if (s->speeds_stale) s->refresh_speeds();
         const bool one = s->one_launch();
         for (int k = 0; k < nsteps; ++k) {  // the launches of device_step(k), with events between
             hipEvent_t* e = &ev[4 * (size_t)k];
@@ -1086,6 +1105,7 @@ int nest_hip_sim2d_pack(nest_sim2d* s, const double* src, double* dst, int direc
     for (const auto& d : s->patches_h) max_nx = std::max(max_nx, d.nx);
     s->L().pack(src, dst, s->patches_d, (int)s->patches_h.size(), max_nx, s->geoms_d, s->goff_d, direction,
                 s->stream);
+    if (direction == 1) s->speeds_stale = true;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -1175,6 +1195,7 @@ int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
         }
     }
     s->fill(s->U);
+    s->speeds_stale = true;
     s->roll_kept = nkeep;
     s->roll_K = nsteps;
     s->roll_m = m;
@@ -1289,6 +1310,10 @@ struct nest_solver1d {
     nest_run1d_state run_h{};
     hipStream_t own_stream = nullptr, stream = nullptr;
     int k_next = 0;  // index of the next fused step of start() / advance() (buffer and slot-set parity)
+    // the state changed outside a fused step since the speeds the next one
+    // reads were reduced (upload, caller-dt stages or steps): the next
+    // advance reduces them again first
+    bool speeds_stale = false;
     int step_kernel = 0;  // fused step kernel: 0 automatic, 1 LDS tile, 2 lanes
     hipGraph_t graph = nullptr;  // kBatch device-dt steps, replayed by run()
     hipGraphExec_t graph_exec = nullptr;
@@ -1330,6 +1355,15 @@ struct nest_solver1d {
         a.slots_clr = slots_d + (size_t)((k + 1) % 3) * kSet;
         a.segs = step1d_segs(n, step_kernel);
         L().step1d(lim, a, stream);
+    }
+    // the speeds of the current state (in U) into the read set of step k_next
+    void refresh_speeds() {
+        unsigned long long* set = slots_d + (size_t)((k_next + 2) % 3) * kSet;
+        L().reset_speeds(ps_d, set, 1, stream);
+        stage1d_args a = args(0.0);
+        a.slots = set;
+        L().speeds1d(a, stream);
+        speeds_stale = false;
     }
     int ensure_graph() {
         if (graph_exec) return NEST_OK;
@@ -1396,6 +1430,7 @@ int nest_hip_solver1d_upload(nest_solver1d* s, const double* host) {
     if (!s || !host) return fail(NEST_ERR_INVALID, "null handle or host");
     NEST_HIP_TRY(hipMemcpyAsync(s->U, host, 3 * (size_t)(s->n + 4) * sizeof(double), hipMemcpyHostToDevice, s->stream));
     NEST_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->speeds_stale = true;
     return NEST_OK;
 }
 
@@ -1422,6 +1457,7 @@ int nest_hip_solver1d_rk2_step(nest_solver1d* s, double dt) {
     if (!(dt > 0.0)) return fail(NEST_ERR_INVALID, "dt must be > 0");
     s->step(dt);
     s->L().fill1d(s->U, s->n, s->bc, nullptr, s->stream);  // hand out a halo-complete state
+    s->speeds_stale = true;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -1443,6 +1479,7 @@ static int solver1d_begin(nest_solver1d* s, double cfl, double t_end) {
     a0.slots = s->slots_d + 2 * nest_solver1d::kSet;
     s->L().speeds1d(a0, s->stream);
     s->k_next = 0;
+    s->speeds_stale = false;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -1487,6 +1524,7 @@ int nest_hip_solver1d_stage(nest_solver1d* s, int stage, double dt) {
     if (!s) return fail(NEST_ERR_INVALID, "null handle");
     if ((stage != 1 && stage != 2) || !(dt > 0.0)) return fail(NEST_ERR_INVALID, "stage must be 1|2, dt > 0");
     s->L().stage1d(stage, s->lim, s->args(dt), s->stream);
+    s->speeds_stale = true;
     NEST_HIP_TRY(hipGetLastError());
     return NEST_OK;
 }
@@ -1511,6 +1549,7 @@ int nest_hip_solver1d_advance(nest_solver1d* s, int nsteps) {
     if (!s) return fail(NEST_ERR_INVALID, "null handle");
     if (nsteps < 0) return fail(NEST_ERR_INVALID, "nsteps < 0");
     const size_t bytes = 3 * (size_t)(s->n + 4) * sizeof(double);
+    if (nsteps > 0 && s->speeds_stale) s->refresh_speeds();
     // fused step k reads the buffer of parity k (U for even k, U1 for odd k);
     // the handle's state is in U between calls
     if (s->k_next & 1)

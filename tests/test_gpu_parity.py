@@ -412,9 +412,11 @@ def test_2d_device_dt_mixed_sequence(oracle):
 @pytest.mark.parametrize("kernel", STAGE_KERNELS)
 def test_2d_host_dt_steps_between_device_dt_steps(oracle, kernel):
     """Device-dt steps, then an RK2 step with the caller's dt (compute_dt of
-    the current state), then device-dt steps again (exact numerics, graph
-    and plain launches): every dt and state equal the oracle's. The host-dt
-    step leaves exactly the new state's speeds for the next device step."""
+    the current state), then device-dt steps again, then another state
+    uploaded and more device-dt steps (exact numerics, graph and plain
+    launches): every dt and state equal the oracle's. The host-dt step
+    leaves exactly the new state's speeds for the next device step; after
+    the upload the next advance reduces them from the uploaded state."""
     nx, ny, npx, npy = 72, 56, 2, 2
     print_case("2d host dt between device dt", kernel)
     sim = Sim2D.mesh(nx, ny, npx, npy)
@@ -437,6 +439,15 @@ def test_2d_host_dt_steps_between_device_dt_steps(oracle, kernel):
     st = sim.status()
     assert st["steps"][0] == 8 and st["dt"][0] == dts[-1], (st, dts)
     assert np.array_equal(sim.gather(sim.download()), sim.gather(ref))
+    # another state uploaded mid-run: the next steps' dt come from it
+    V = sim.scatter(P.sod2d(nx, ny, jitter=1e-3, seed=12, x_disc=0.3))
+    oracle.mesh_fill(V, m)
+    sim.upload(V)
+    sim.advance(6, graph=True)
+    _, dts = oracle.mesh_run(V, m, 6, dx, dy, cfl=0.4)
+    st = sim.status()
+    assert st["steps"][0] == 14 and st["dt"][0] == dts[-1], (st, dts)
+    assert np.array_equal(sim.gather(sim.download()), sim.gather(V))
     sim.close()
 
 
@@ -768,6 +779,33 @@ def test_1d_start_advance_and_stages(oracle, bc, kernel):
     oracle.rk2_step_1d(ref, n, dt, dx, bc=bc)
     oracle.fill_1d(ref, n, bc=bc)
     assert np.array_equal(s.download(), ref)
+    # a device-dt run continued after the state changed outside it (a
+    # caller-dt step, then an upload): each next step's dt comes from the
+    # current state
+    s.upload(U)
+    s.start(cfl)
+    s.advance(2)
+    ref = U.copy()
+    for _ in range(2):
+        oracle.rk2_step_1d(ref, n, cfl * dx / oracle.max_speed_1d(ref, n), dx, bc=bc)
+    oracle.fill_1d(ref, n, bc=bc)
+    dt = s.compute_dt(cfl)
+    s.rk2_step(0.5 * dt)
+    oracle.rk2_step_1d(ref, n, 0.5 * dt, dx, bc=bc)
+    oracle.fill_1d(ref, n, bc=bc)
+    s.advance(3)
+    for _ in range(3):
+        oracle.rk2_step_1d(ref, n, cfl * dx / oracle.max_speed_1d(ref, n), dx, bc=bc)
+    oracle.fill_1d(ref, n, bc=bc)
+    assert np.array_equal(s.download(), ref), bc
+    V = P.sod1d(n, x_disc=0.3)  # another tube, uploaded mid-run
+    oracle.fill_1d(V, n, bc=bc)
+    s.upload(V)
+    s.advance(2)
+    for _ in range(2):
+        oracle.rk2_step_1d(V, n, cfl * dx / oracle.max_speed_1d(V, n), dx, bc=bc)
+    oracle.fill_1d(V, n, bc=bc)
+    assert np.array_equal(s.download(), V), bc
     s.close()
 
 
