@@ -506,6 +506,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--two-stage", action="store_true",
+                    help="time the two-launch step (stage 1 writes U1, stage 2 reads it) instead of the one-launch step")
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--wpb", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -533,6 +535,8 @@ def main():
     log(f"bench: building 4096^2 Sod state ({args.mode})")
     G = P.sod2d(N, N, jitter=1e-3, seed=2)
     sim = Sim2D.mesh(N, N, NP, NP, mode=args.mode)
+    if args.two_stage:
+        sim.set_step_kernel("two_stage")
     U = sim.scatter(G)
     tiling = None
     if args.rows or args.wpb:
