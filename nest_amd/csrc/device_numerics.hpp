@@ -78,13 +78,16 @@ __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 // For a*b > 0 the reference's (|a| < |b|) ? a : b is the smaller magnitude
 // with the common sign: copysign(min(|a|, |b|), a) returns the same bits
 // (equal magnitudes give b's bits, as the reference does).
+// Fast numerics take the median of (a, b, 0): the smaller magnitude when a
+// and b share a sign, else 0 (the same value; a zero result may carry either
+// sign). Four hardware min/max, no compare, no selects.
 __device__ __forceinline__ double minmod(double a, double b) {
 #if NEST_FAST
-    double r = __builtin_copysign(__builtin_fmin(__builtin_fabs(a), __builtin_fabs(b)), a);
+    return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
 #else
     double r = (__builtin_fabs(a) < __builtin_fabs(b)) ? a : b;
-#endif
     return (a * b <= 0.0) ? 0.0 : r;
+#endif
 }
 template <int LIM>
 __device__ __forceinline__ double limiter(double l, double r) {
@@ -115,16 +118,18 @@ __device__ __forceinline__ prim4 prim2d(double r_in, double mu, double mv, doubl
     prim4 w;
     double rho = dmax(r_in, 1e-10);
 #if NEST_FAST
+    // rho (u^2 + v^2) = mu u + mv v: the same value with two fewer products
     double ir = rcp(rho);
     double u = mu * ir;
     double v = mv * ir;
+    w.p = dmax(e.gm1 * __builtin_fma(-0.5, mu * u + mv * v, E), 1e-10);
 #else
     double u = mu / rho;
     double v = mv / rho;
-#endif
     double ke = 0.5 * rho * (u * u + v * v);
-    w.r = rho;
     w.p = dmax(e.gm1 * (E - ke), 1e-10);
+#endif
+    w.r = rho;
     w.u = xsweep ? u : v;
     w.q = xsweep ? mv : mu;
     return w;
@@ -212,14 +217,29 @@ __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const 
 #endif
 #if NEST_FAST && NEST_HLL_CLAMP
     if constexpr (CLAMP && PASSIVE) {
+        // the weights SR+/(SR+ - SL-), SL-/(SR+ - SL-) and their product
+        // term formed once; enthalpy rho H = p gamma/(gamma-1) + rho u^2/2
+        // gives the energy flux and E (= rho H - p) with one product less
         const double SLm = dmin(SL, 0.0), SRp = dmax(SR, 0.0);
         const double inv = rcp(SRp - SLm);
-        const double s2 = SLm * SRp;
-        F[0] = (SRp * FL0 - SLm * FR0 + s2 * (R.r - L.r)) * inv;
-        F[1] = (SRp * FL1 - SLm * FR1 + s2 * (R.r * R.u - L.r * L.u)) * inv;
-        F[2] = (SRp * FL2 - SLm * FR2 + s2 * (ER - EL)) * inv;
-        const double ui = (SRp * L.u - SLm * R.u) * inv;
+        const double wr = SRp * inv, wl = SLm * inv, ws = SLm * wr;
+        const double g1 = e.gamma * e.inv_gm1;
+        const double mL = L.r * L.u, mR = R.r * R.u;
+        const double kL = mL * L.u, kR = mR * R.u;
+        const double HL = __builtin_fma(0.5, kL, L.p * g1), HR = __builtin_fma(0.5, kR, R.p * g1);
+        F[0] = __builtin_fma(ws, R.r - L.r, __builtin_fma(-wl, mR, wr * mL));
+        F[1] = __builtin_fma(ws, mR - mL, __builtin_fma(-wl, kR + R.p, wr * (kL + L.p)));
+        F[2] = __builtin_fma(ws, (HR - HL) - (R.p - L.p), __builtin_fma(-wl, HR * R.u, wr * (HL * L.u)));
+        const double ui = __builtin_fma(-wl, R.u, wr * L.u);
         F[3] = ui * ((ui >= 0.0) ? L.q : R.q);
+        (void)EL;
+        (void)ER;
+        (void)FL0;
+        (void)FL1;
+        (void)FL2;
+        (void)FR0;
+        (void)FR1;
+        (void)FR2;
         return;
     }
 #endif

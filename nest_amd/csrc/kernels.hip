@@ -88,8 +88,12 @@ __device__ __forceinline__ void cell_speeds2d(double r, double mu, double mv, do
     double rho = dmax(r, 1e-10);
     double u = ddiv(mu, rho);
     double v = ddiv(mv, rho);
+#if NEST_FAST
+    double p = dmax(e.gm1 * __builtin_fma(-0.5, mu * u + mv * v, E), 1e-10);
+#else
     double ke = 0.5 * rho * (u * u + v * v);
     double p = dmax(e.gm1 * (E - ke), 1e-10);
+#endif
     double c = sound_speed(rho, p, e);
     sx = __builtin_fabs(u) + c;
     sy = __builtin_fabs(v) + c;
