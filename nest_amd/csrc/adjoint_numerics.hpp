@@ -129,15 +129,118 @@ __device__ __forceinline__ void prim2d_rev(const prim4& W, bool xsweep, const pr
 }
 
 // ---------------------------------------------------------------------------
+// Face flux reverse in the clamped form of the HLLE flux (the form the fast
+// forward evaluates): with SL- = min(SL, 0), SR+ = max(SR, 0),
+// inv = 1/(SR+ - SL-) and the weights wr = SR+ inv, wl = SL- inv,
+// ws = SL- SR+ inv,
+//   F_k = wr FL_k - wl FR_k + ws (UR_k - UL_k),  u_int = wr uL - wl uR,
+// which is F_L when SL >= 0, F_R when SR <= 0 and the HLLE average between.
+// Its derivative is the derivative of the branch the primal takes: in the
+// upwind branches one weight is 0 and the other 1, and the speed cotangents
+// vanish (SL- is constant for SL >= 0, SR+ for SR <= 0). With
+// A = sum Fb_k FL_k, B = sum Fb_k FR_k, C = sum Fb_k (UR_k - UL_k) (the
+// interface velocity counted in A and B with its cotangent):
+//   d/dSL- = wr ((A - B) inv + C wr),  d/dSR+ = wl ((B - A) inv - C wl).
+// One formula for every face: no per-branch selects. Evaluated with the
+// numerics mode's division and sound speed (fast: c = (gamma p) y with
+// y = rsq(gamma p rho), so dc/dp = gamma y / 2 and dc/drho = -c^2 y / 2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void face_flux_rev_clamped(const prim4& L, const prim4& R, const eos_c& e,
+                                                      const double Fb[4], prim4& Lb, prim4& Rb) {
+#if NEST_FAST
+    const double gpL = e.gamma * L.p, gpR = e.gamma * R.p;
+    const double yL = rsq(gpL * L.r), yR = rsq(gpR * R.r);
+    const double cL = gpL * yL, cR = gpR * yR;
+    const double cLp = 0.5 * e.gamma * yL, cRp = 0.5 * e.gamma * yR;
+    const double cLr = -0.5 * cL * cL * yL, cRr = -0.5 * cR * cR * yR;
+    const double igm1 = e.inv_gm1;
+#else
+    const double cL = sound_speed(L.r, L.p, e), cR = sound_speed(R.r, R.p, e);
+    const double cLp = 0.5 * ddiv(cL, L.p), cRp = 0.5 * ddiv(cR, R.p);
+    const double cLr = -0.5 * ddiv(cL, L.r), cRr = -0.5 * ddiv(cR, R.r);
+    const double igm1 = 1.0 / e.gm1;
+#endif
+    const double aL = L.u - cL, aR = R.u - cR;
+    const double bL = L.u + cL, bR = R.u + cR;
+    const bool sl_from_r = aR < aL;  // SL = min(aL, aR), std::min order
+    const bool sr_from_r = bL < bR;  // SR = max(bL, bR), std::max order
+    const double SL = sl_from_r ? aR : aL;
+    const double SR = sr_from_r ? bR : bL;
+    const double SLm = dmin(SL, 0.0), SRp = dmax(SR, 0.0);
+    const double inv = ddiv(1.0, SRp - SLm);
+    const double wr = SRp * inv, wl = SLm * inv, ws = SLm * wr;
+    // physical fluxes (r u, r u^2 + p, r H u) and jumps (dr, d(r u), dE),
+    // r H = E + p = gamma/(gamma-1) p + r u^2 / 2
+    const double g1 = e.gamma * igm1;
+    const double mL = L.r * L.u, mR = R.r * R.u;
+    const double kL = mL * L.u, kR = mR * R.u;
+    const double HL = 0.5 * kL + g1 * L.p, HR = 0.5 * kR + g1 * R.p;
+    const double FL1 = kL + L.p, FR1 = kR + R.p;
+    const double FL2 = HL * L.u, FR2 = HR * R.u;
+    const double dU0 = R.r - L.r, dU1 = mR - mL, dU2 = (HR - HL) - (R.p - L.p);
+    // passive flux F3 = u_int * (u_int >= 0 ? qL : qR)  (euler2d.hpp:27-33)
+    const double ui = wr * L.u - wl * R.u;
+    const bool upw = ui >= 0.0;
+    const double uib = Fb[3] * (upw ? L.q : R.q);
+    Lb.q += upw ? Fb[3] * ui : 0.0;
+    Rb.q += upw ? 0.0 : Fb[3] * ui;
+    const double A = Fb[0] * mL + Fb[1] * FL1 + Fb[2] * FL2 + uib * L.u;
+    const double B = Fb[0] * mR + Fb[1] * FR1 + Fb[2] * FR2 + uib * R.u;
+    const double C = Fb[0] * dU0 + Fb[1] * dU1 + Fb[2] * dU2;
+    const double SLb = (SL < 0.0) ? wr * ((A - B) * inv + C * wr) : 0.0;
+    const double SRb = (SR > 0.0) ? wl * ((B - A) * inv - C * wl) : 0.0;
+    // cotangents of FL_k (wr Fb_k), FR_k (-wl Fb_k) and the jumps (ws Fb_k)
+    const double fl0 = wr * Fb[0], fl1 = wr * Fb[1], fl2 = wr * Fb[2];
+    const double fr0 = -wl * Fb[0], fr1 = -wl * Fb[1], fr2 = -wl * Fb[2];
+    const double du0 = ws * Fb[0], du1 = ws * Fb[1], du2 = ws * Fb[2];
+    // left state: m = r u, k = m u, r H = k / 2 + g1 p
+    const double hLb = fl2 * L.u - du2;
+    const double kLb = fl1 + 0.5 * hLb;
+    const double mLb = fl0 - du1;
+    double uLb = fl2 * HL + mLb * L.r + 2.0 * kLb * mL + uib * wr;
+    double rLb = L.u * (mLb + kLb * L.u) - du0;
+    double pLb = fl1 + g1 * hLb + du2;
+    // right state
+    const double hRb = fr2 * R.u + du2;
+    const double kRb = fr1 + 0.5 * hRb;
+    const double mRb = fr0 + du1;
+    double uRb = fr2 * HR + mRb * R.r + 2.0 * kRb * mR - uib * wl;
+    double rRb = R.u * (mRb + kRb * R.u) + du0;
+    double pRb = fr1 + g1 * hRb - du2;
+    // Davis speeds and the sound speeds under them
+    const double aLb = sl_from_r ? 0.0 : SLb, aRb = sl_from_r ? SLb : 0.0;
+    const double bLb = sr_from_r ? 0.0 : SRb, bRb = sr_from_r ? SRb : 0.0;
+    uLb += aLb + bLb;
+    uRb += aRb + bRb;
+    const double cLb = bLb - aLb, cRb = bRb - aRb;
+    pLb += cLb * cLp;
+    rLb += cLb * cLr;
+    pRb += cRb * cRp;
+    rRb += cRb * cRr;
+    Lb.r += rLb;
+    Lb.u += uLb;
+    Lb.p += pLb;
+    Rb.r += rRb;
+    Rb.u += uRb;
+    Rb.p += pRb;
+}
+
+// ---------------------------------------------------------------------------
 // face flux reverse: face_flux<true> (HLLE with Davis speeds on (rho, u_n, p)
 // plus the passive transverse-momentum flux). Fb = cotangent of F[0..3];
 // accumulates the cotangents of the two face states into Lb, Rb.
-// Branch-free: the subsonic (HLLE-average) terms are formed for every face
-// (SR > SL always, since c > 0) and the branch the primal takes selects
-// which terms are passed on, so a wave has no divergent paths here.
+// Fast numerics: the clamped form above (the fast forward's own form).
+// Exact numerics: branch-free over the reference's branches: the subsonic
+// (HLLE-average) terms are formed for every face (SR > SL always, since
+// c > 0) and the branch the primal takes selects which terms are passed on,
+// so a wave has no divergent paths here.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void face_flux_rev(const prim4& L, const prim4& R, const eos_c& e, const double Fb[4],
                                               prim4& Lb, prim4& Rb) {
+#if NEST_FAST
+    face_flux_rev_clamped(L, R, e, Fb, Lb, Rb);
+    return;
+#endif
     // ---- primal recompute, as face_flux<true>
 #if NEST_FAST
     // c = (gamma p) y with y = rsq(gamma p rho), as sound_speed; y also gives

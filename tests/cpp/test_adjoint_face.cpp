@@ -4,6 +4,7 @@
 //   <Wb, J dU> == <J^T Wb, dU>   for get_prim_2d (both sweep directions)
 //   <sb, ds>   == sb*(cl dl + cr dr)   for every limiter
 //   <Fb, J dLR> == <Lb, dL> + <Rb, dR>   for the HLLE + passive face flux
+//     (branch form, and the clamped form the fast numerics run)
 // over random states that visit every branch (floors, upwind / subsonic
 // HLLE, both Davis speed choices, both passive-flux signs). Exit status 0 iff
 // every relative mismatch is below 1e-12.
@@ -133,6 +134,16 @@ static void test_face(int n) {
             scale += std::fabs(lb[k] * dL[k]) + std::fabs(rb[k] * dR[k]);
         }
         check("face", i, lhs, rhs, scale);
+        // the clamped-form reverse (what fast numerics run), same contract
+        ne::prim4 Lc = ne::zero4(), Rc = ne::zero4();
+        ne::face_flux_rev_clamped(Lp, Rp, EOS, Fb, Lc, Rc);
+        const double lc[4] = {Lc.r, Lc.u, Lc.p, Lc.q}, rc[4] = {Rc.r, Rc.u, Rc.p, Rc.q};
+        double rhs_c = 0.0, scale_c = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            rhs_c += lc[k] * dL[k] + rc[k] * dR[k];
+            scale_c += std::fabs(lc[k] * dL[k]) + std::fabs(rc[k] * dR[k]);
+        }
+        check("face clamped", i, lhs, rhs_c, scale_c);
         ++branch_count[kind];
     }
     std::printf("face cases: %d subsonic, %d right-upwind, %d left-upwind\n", branch_count[0], branch_count[1],
