@@ -71,16 +71,41 @@ def cpu_baseline(budget_s: float):
     return {"value": round(val, 3), "unit": "Mcell-updates/s", "cores": nthr, "kind": "port",
             "sample": f"{steps} RK2 step(s) of the same 4096^2 16x16-patch Sod workload (dt from the "
                       f"state each step), {el:.1f} s wall on {nthr} OpenMP threads (OMP_PROC_BIND=close, "
-                      f"OMP_PLACES=cores) of {os.cpu_count()} CPUs ({cpu_model()}); oracle built -O3 "
-                      f"-ffp-contract=off for the generic x86-64 target"}
+                      f"OMP_PLACES=cores) of {os.cpu_count()} CPUs ({cpu_model()}); oracle built "
+                      f"{res['build']}"}
+
+
+ORACLE_FLAGS = ["-std=c++20", "-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math"]
+
+
+def _native_oracle(tmp: str):
+    """The oracle compiled on this host with -march=native (a few seconds of
+    g++), so the baseline uses the host's own instruction set; None where no
+    compiler is available."""
+    import subprocess
+
+    src = os.path.join(ROOT, "oracle", "oracle_capi.cpp")
+    out = os.path.join(tmp, "liboracle_native.so")
+    try:
+        r = subprocess.run(["g++", *ORACLE_FLAGS, "-march=native", "-shared", "-o", out, src],
+                           capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    return out if r.returncode == 0 else None
 
 
 def _cpu_baseline_child(budget_s: float) -> None:
     """The timed oracle loop of cpu_baseline (child process)."""
+    import tempfile
+
     from nest_amd import problems as P
     from oracle.oracle import MeshSpec, Oracle
 
-    o = Oracle()
+    with tempfile.TemporaryDirectory(prefix="nest_cpu_") as tmp:
+        native = _native_oracle(tmp)
+        o = Oracle(native) if native else Oracle()  # loaded: the file may go
+    build = " ".join(ORACLE_FLAGS[1:2] + ORACLE_FLAGS[4:5]) + (
+        " -march=native on this host" if native else " for the generic x86-64 target (no compiler here)")
     m = MeshSpec(N, N, NP, NP)
     U = m.to_patches(P.sod2d(N, N, jitter=1e-3, seed=2))
     o.mesh_fill(U, m)
@@ -92,7 +117,7 @@ def _cpu_baseline_child(budget_s: float) -> None:
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 20:
             break
-    print(json.dumps({"steps": steps, "seconds": el, "threads": o.get_threads()}), flush=True)
+    print(json.dumps({"steps": steps, "seconds": el, "threads": o.get_threads(), "build": build}), flush=True)
 
 
 def copy_peak_gbs(gib: int = 2, reps: int = 10) -> dict:
