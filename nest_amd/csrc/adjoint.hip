@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         u[3] = q[192];
     };
     for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
-        const nest_item w = a.items[it];
+        const nest_item w = uniform_item(a.items[it]);
         const nest_patch_desc pd = a.patches[w.patch];
         const nest_prob_state& ps = a.probs[pd.prob];
         const double dtdx = a.dt / ps.dx;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJX_W
     const int nw = gridDim.x * wpb;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
-        const nest_item w = a.items[it];
+        const nest_item w = uniform_item(a.items[it]);
         const nest_patch_desc pd = a.patches[w.patch];
         const nest_prob_state& ps = a.probs[pd.prob];
         const double dtdx = a.dt / ps.dx;

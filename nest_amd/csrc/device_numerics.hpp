@@ -6,11 +6,12 @@
 //     (examples/sod1d/euler1d.hpp:27-242, examples/sod2d/euler2d.hpp:20-64),
 //     with correctly rounded division and square root, so results are
 //     bitwise those of the CPU path.
-//   NEST_FAST=1 (FMA contraction on): divisions become reciprocal-multiplies
-//     (hardware v_rcp_f64 refined by two Newton steps), per-face quotients
-//     share one reciprocal, and the EOS uses 1/(gamma-1). Differences from
-//     the CPU path are a few ulp per operation; the per-step tolerance is
-//     1e-12 relative.
+//   NEST_FAST=1 (FMA contraction on, NaNs assumed absent, IEEE mode off):
+//     divisions become reciprocal-multiplies (hardware v_rcp_f64 refined by
+//     one Newton step), per-face quotients share one reciprocal, the EOS uses
+//     1/(gamma-1) and the limiters hardware min / max. Differences from the
+//     CPU path are a few ulp per operation; the per-step tolerance is 1e-12
+//     relative.
 #pragma once
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -71,6 +72,19 @@ __device__ __forceinline__ double ddiv(double a, double b) { return a * rcp(b); 
 __device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
 #endif
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+
+// NaN and infinity tests on the bit pattern. The fast kernels are compiled
+// with NaNs assumed absent (so min / max need no NaN quieting), under which
+// x != x folds to false; these integer tests hold under any floating-point
+// assumption.
+__device__ __forceinline__ bool bits_nan(double x) {
+    return ((unsigned long long)__builtin_bit_cast(long long, x) & 0x7fffffffffffffffull) > 0x7ff0000000000000ull;
+}
+__device__ __forceinline__ bool bits_inf(double x) {
+    return ((unsigned long long)__builtin_bit_cast(long long, x) & 0x7fffffffffffffffull) == 0x7ff0000000000000ull;
+}
+// dt usable for a step: finite and positive
+__device__ __forceinline__ bool dt_ok(double dt) { return !bits_nan(dt) && !bits_inf(dt) && dt > 0.0; }
 
 // ---------------------------------------------------------------------------
 // limiters — euler1d.hpp:112-137
@@ -345,6 +359,14 @@ __device__ __forceinline__ void wait_lgkmcnt0() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 __device__ __forceinline__ void compiler_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+// A work item read by a whole wave (wave-uniform index), its fields marked
+// wave-uniform: the compiler keeps them, and the march bounds derived from
+// them, in scalar registers whatever else it infers.
+__device__ __forceinline__ nest_item uniform_item(const nest_item& w) {
+    return {__builtin_amdgcn_readfirstlane(w.patch), __builtin_amdgcn_readfirstlane(w.c0),
+            __builtin_amdgcn_readfirstlane(w.r0), __builtin_amdgcn_readfirstlane(w.r1)};
+}
 
 __device__ __forceinline__ const char* shfl_cptr(const void* p, int src) {
     return reinterpret_cast<const char*>(__shfl(reinterpret_cast<long long>(p), src));

@@ -82,6 +82,27 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// Slot value (bit pattern) of a wave speed. Fast numerics floor density and
+// pressure with hardware max, which drops a NaN operand, so a NaN or infinity
+// in a cell's density or energy need not reach its speed: such a cell
+// reports the all-ones pattern, which ranks above every speed in the slots'
+// u64 maxima and reads back as a NaN (the run then stops with a non-finite
+// dt, as it does in exact numerics, where the floors pass a NaN on). Tested
+// on the exponent bits, which no floating-point assumption of the compiler
+// can fold away. Momenta reach the speeds without a floor.
+__device__ __forceinline__ unsigned long long speed_slot_bits(double s, double rho, double E) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(s);
+#if NEST_FAST
+    const unsigned hr = (unsigned)__double2hiint(rho) & 0x7fffffffu;
+    const unsigned hE = (unsigned)__double2hiint(E) & 0x7fffffffu;
+    b = (hr > hE ? hr : hE) >= 0x7ff00000u ? ~0ull : b;
+#else
+    (void)rho;
+    (void)E;
+#endif
+    return b;
+}
+
 // |u|+c and |v|+c of one cell, as compute_dt_2d (euler2d.hpp:254-263)
 __device__ __forceinline__ void cell_speeds2d(double r, double mu, double mv, double E, const eos_c& e,
                                               double& sx, double& sy) {
@@ -127,7 +148,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
         u[3] = q[192];
     };
     for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
-        const nest_item w = a.items[it];
+        const nest_item w = uniform_item(a.items[it]);
         const nest_patch_desc pd = a.patches[w.patch];
         nest_prob_state* ps = a.probs + pd.prob;
         if (!(a.host_dt > 0.0) && ps->done) continue;
@@ -293,7 +314,7 @@ __global__ __launch_bounds__(256) NEST_STAGE_OCCUPANCY void stage2d_kernel(stage
                     o3 = 0.5 * (U0[3] + o3);
                     double sx, sy2;
                     cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
-                    unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+                    unsigned long long bx = speed_slot_bits(sx, o0, o3);
                     unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
                     smx = (store && bx > smx) ? bx : smx;
                     smy = (store && by > smy) ? by : smy;
@@ -417,7 +438,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int hl = lane >> 1;
     const int part = (lane & 1) * 4;
     for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
-        const nest_item w = a.items[it];
+        const nest_item w = uniform_item(a.items[it]);
         const nest_patch_desc pd = a.patches[w.patch];
         nest_prob_state* ps = a.probs + pd.prob;
         if (!(a.host_dt > 0.0) && ps->done) continue;
@@ -626,7 +647,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 o3 = 0.5 * (z3 + o3);
                 double sx, sy2;
                 cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
-                unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+                unsigned long long bx = speed_slot_bits(sx, o0, o3);
                 unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
                 smx = (store && bx > smx) ? bx : smx;
                 smy = (store && by > smy) ? by : smy;
@@ -708,7 +729,7 @@ __device__ inline int step_dt(const nest_prob_state& s, double t, unsigned long 
         dt = s.cfl * dmin(dtx, dty);
     }
     dt = dmin(dt, s.t_end - t);
-    if (!(dt > 0.0) || dt != dt || dt == __builtin_inf() || sx != sx || sy != sy) return 2;
+    if (!dt_ok(dt) || bits_nan(sx) || bits_nan(sy)) return 2;
     return 0;
 }
 constexpr int kStepRing = 8;
@@ -729,7 +750,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
     const int hl = lane >> 1;
     const int part = (lane & 1) * 4;
     for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + wave; it < a.nitems; it += nw) {
-        const nest_item w = a.items[it];
+        const nest_item w = uniform_item(a.items[it]);
         const nest_patch_desc pd = a.patches[w.patch];
         nest_prob_state* ps = a.probs + pd.prob;
         const nest_prob_geom G = a.geoms[pd.prob];
@@ -950,7 +971,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
             o3 = 0.5 * (z[3] + o3);
             double sx, sy2;
             cell_speeds2d(o0, o1, o2, o3, a.eos, sx, sy2);
-            const unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+            const unsigned long long bx = speed_slot_bits(sx, o0, o3);
             const unsigned long long by = (unsigned long long)__double_as_longlong(sy2);
             smx = (store && bx > smx) ? bx : smx;
             smy = (store && by > smy) ? by : smy;
@@ -1116,7 +1137,7 @@ __global__ __launch_bounds__(256) void speeds2d_kernel(stage_args a) {
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
     for (int it = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); it < a.nitems; it += nw) {
-        const nest_item w = a.items[it];
+        const nest_item w = uniform_item(a.items[it]);
         const nest_patch_desc pd = a.patches[w.patch];
         nest_prob_state* ps = a.probs + pd.prob;
         const int nye = pd.ny + 4;
@@ -1129,8 +1150,9 @@ __global__ __launch_bounds__(256) void speeds2d_kernel(stage_args a) {
             for (int i = w.r0; i < w.r1; ++i) {
                 const double* p = p0 + (size_t)(i + 2) * nye;
                 double sx, sy;
-                cell_speeds2d(p[0], p[vs], p[2 * (size_t)vs], p[3 * (size_t)vs], a.eos, sx, sy);
-                unsigned long long bx = (unsigned long long)__double_as_longlong(sx);
+                const double r = p[0], E = p[3 * (size_t)vs];
+                cell_speeds2d(r, p[vs], p[2 * (size_t)vs], E, a.eos, sx, sy);
+                unsigned long long bx = speed_slot_bits(sx, r, E);
                 unsigned long long by = (unsigned long long)__double_as_longlong(sy);
                 smx = bx > smx ? bx : smx;
                 smy = by > smy ? by : smy;
@@ -1287,7 +1309,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
             for (int k = 0; k < 3; ++k) o[k] = 0.5 * (out[k * ne + e] + o[k]);
             prim4 w = prim1d(o[0], o[1], o[2], a.eos);
             double s = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
-            sm = (unsigned long long)__double_as_longlong(s);
+            sm = speed_slot_bits(s, o[0], o[2]);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
@@ -1385,7 +1407,7 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
             } else {
                 dtn = P.fixed_dt > 0.0 ? P.fixed_dt : P.cfl * P.dx / sx;
                 dtn = dmin(dtn, P.t_end - t);
-                if (!(dtn > 0.0) || dtn != dtn || dtn == __builtin_inf() || sx != sx) done = 2;
+                if (!dt_ok(dtn) || bits_nan(sx)) done = 2;
             }
             s_go = !done;
             s_dt = dtn;
@@ -1483,7 +1505,7 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
         for (int k = 0; k < 3; ++k) o[k] = 0.5 * (in[k * ne + e] + o[k]);
         prim4 w = prim1d(o[0], o[1], o[2], a.eos);
         double sp = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
-        sm = (unsigned long long)__double_as_longlong(sp);
+        sm = speed_slot_bits(sp, o[0], o[2]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) out[k * ne + e] = o[k];
     }
@@ -1549,7 +1571,7 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
     } else {
         dt = P.fixed_dt > 0.0 ? P.fixed_dt : P.cfl * P.dx / sx;
         dt = dmin(dt, P.t_end - t_in);
-        if (!(dt > 0.0) || dt != dt || dt == __builtin_inf() || sx != sx) done = 2;
+        if (!dt_ok(dt) || bits_nan(sx)) done = 2;
     }
     if (wid == 0) {
         if (lane == 0 && done >= 0) {
@@ -1618,7 +1640,7 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
             const double o0 = 0.5 * (z0 + c0), o1 = 0.5 * (z1 + c1), o2 = 0.5 * (z2 + c2);
             const prim4 w = prim1d(o0, o1, o2, a.eos);
             const double sp = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
-            const unsigned long long b = (unsigned long long)__double_as_longlong(sp);
+            const unsigned long long b = speed_slot_bits(sp, o0, o2);
             sm = b > sm ? b : sm;
             const int e = g + 2;
             out[e] = o0;
@@ -1643,9 +1665,10 @@ __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
     const int e = 2 + blockIdx.x * 256 + threadIdx.x;
     unsigned long long sm = 0;
     if (e < n + 2) {
-        prim4 w = prim1d(a.U[e], a.U[ne + e], a.U[2 * ne + e], a.eos);
+        const double r = a.U[e], E = a.U[2 * ne + e];
+        prim4 w = prim1d(r, a.U[ne + e], E, a.eos);
         double s = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
-        sm = (unsigned long long)__double_as_longlong(s);
+        sm = speed_slot_bits(s, r, E);
     }
     sm = wave_max_u64(sm);
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
