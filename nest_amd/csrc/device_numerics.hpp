@@ -185,10 +185,19 @@ __device__ __forceinline__ double sound_speed(double rho, double p, const eos_c&
     return dsqrt(ddiv(e.gamma * p, rho));
 #endif
 }
-// 1D variant: the 1D pressure clamp is p >= 0 (euler1d.hpp:43), so p may be
-// exactly 0 where the reciprocal square root form would give 0*inf.
+// 1D variant. The cell pressure is clamped to p >= 0 (euler1d.hpp:43), so p
+// may be exactly 0, and a face state of the unlimited scheme may have p < 0
+// together with rho < 0 (a positive quotient). Fast numerics use
+// sqrt(gamma p / rho) = |gamma p| rsq(gamma p rho) for either sign of the
+// pair, c = 0 for p = 0 (where the product form would give 0 * inf), and a
+// NaN where the quotient is negative, as sqrt gives.
 __device__ __forceinline__ double sound_speed_1d(double rho, double p, const eos_c& e) {
+#if NEST_FAST
+    const double gp = e.gamma * p;
+    return gp != 0.0 ? __builtin_fabs(gp) * rsq(gp * rho) : 0.0;
+#else
     return dsqrt(ddiv(e.gamma * p, rho));
+#endif
 }
 // total energy — euler1d.hpp:47-51
 __device__ __forceinline__ double total_energy(double rho, double u, double p, const eos_c& e) {
