@@ -216,14 +216,15 @@ __device__ __forceinline__ double total_energy(double rho, double u, double p, c
 // the two (the reference recomputes the same values, :116-118).
 // F[0..2] = HLLE (mass, normal momentum, energy); F[3] = passive flux.
 //
-// CLAMP (fast numerics, 2D only): the three branches in one formula, with
+// CLAMP (fast numerics): the three branches in one formula, with
 // the speeds clamped as SL- = min(SL, 0), SR+ = max(SR, 0):
 //   F = (SR+ F_L - SL- F_R + SL- SR+ (U_R - U_L)) / (SR+ - SL-)
 // is F_L when SL >= 0 (SL- = 0, the quotient SR F_L / SR), F_R when SR <= 0,
 // and the HLLE average otherwise; likewise the interface velocity. It
 // replaces the branch selects by two min/max and differs from the selected
 // value by the rounding of SR * (1/SR) (a few ulp). The denominator is
-// positive: SR - SL >= c_L + c_R > 0, as the 2D floors keep rho, p > 0.
+// positive in 2D: SR - SL >= c_L + c_R > 0, as the 2D floors keep rho, p > 0
+// (1D: see the floor below).
 template <bool PASSIVE, bool CLAMP = false>
 __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const eos_c& e, double F[4]) {
     double cL = PASSIVE ? sound_speed(L.r, L.p, e) : sound_speed_1d(L.r, L.p, e);
@@ -239,12 +240,15 @@ __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const 
 #define NEST_HLL_CLAMP 1
 #endif
 #if NEST_FAST && NEST_HLL_CLAMP
-    if constexpr (CLAMP && PASSIVE) {
+    if constexpr (CLAMP) {
         // the weights SR+/(SR+ - SL-), SL-/(SR+ - SL-) and their product
         // term formed once; enthalpy rho H = p gamma/(gamma-1) + rho u^2/2
-        // gives the energy flux and E (= rho H - p) with one product less
+        // gives the energy flux and E (= rho H - p) with one product less.
+        // 1D allows c = 0 on both sides (p = 0): SR+ - SL- = 0 only when
+        // u = c = 0 on both sides, where every flux is 0 and the floored
+        // denominator gives exactly that.
         const double SLm = dmin(SL, 0.0), SRp = dmax(SR, 0.0);
-        const double inv = rcp(SRp - SLm);
+        const double inv = PASSIVE ? rcp(SRp - SLm) : rcp(dmax(SRp - SLm, 1e-300));
         const double wr = SRp * inv, wl = SLm * inv, ws = SLm * wr;
         const double g1 = e.gamma * e.inv_gm1;
         const double mL = L.r * L.u, mR = R.r * R.u;
@@ -253,8 +257,12 @@ __device__ __forceinline__ void face_flux(const prim4& L, const prim4& R, const 
         F[0] = __builtin_fma(ws, R.r - L.r, __builtin_fma(-wl, mR, wr * mL));
         F[1] = __builtin_fma(ws, mR - mL, __builtin_fma(-wl, kR + R.p, wr * (kL + L.p)));
         F[2] = __builtin_fma(ws, (HR - HL) - (R.p - L.p), __builtin_fma(-wl, HR * R.u, wr * (HL * L.u)));
-        const double ui = __builtin_fma(-wl, R.u, wr * L.u);
-        F[3] = ui * ((ui >= 0.0) ? L.q : R.q);
+        if constexpr (PASSIVE) {
+            const double ui = __builtin_fma(-wl, R.u, wr * L.u);
+            F[3] = ui * ((ui >= 0.0) ? L.q : R.q);
+        } else {
+            F[3] = 0.0;
+        }
         (void)EL;
         (void)ER;
         (void)FL0;

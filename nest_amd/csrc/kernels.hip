@@ -1289,7 +1289,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(stage1d_args a) {
     __syncthreads();
     for (int t = threadIdx.x; t < 257; t += 256) {  // face between cells base-1+t and base+t
         double f[4];
-        face_flux<false>(face_plus(W[t + 1], S[t]), face_minus(W[t + 2], S[t + 1]), a.eos, f);
+        face_flux<false, true>(face_plus(W[t + 1], S[t]), face_minus(W[t + 2], S[t + 1]), a.eos, f);
         F[0][t] = f[0];
         F[1][t] = f[1];
         F[2][t] = f[2];
@@ -1443,7 +1443,7 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     __syncthreads();
     for (int t = threadIdx.x + 1; t < T - 2; t += 256) {
         double f[4];
-        face_flux<false>(face_plus(W(t), S(t)), face_minus(W(t + 1), S(t + 1)), a.eos, f);
+        face_flux<false, true>(face_plus(W(t), S(t)), face_minus(W(t + 1), S(t + 1)), a.eos, f);
         F[0][t] = f[0];
         F[1][t] = f[1];
         F[2][t] = f[2];
@@ -1488,7 +1488,7 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     __syncthreads();
     for (int t = threadIdx.x + 3; t < T - 4; t += 256) {
         double f[4];
-        face_flux<false>(face_plus(W(t), S(t)), face_minus(W(t + 1), S(t + 1)), a.eos, f);
+        face_flux<false, true>(face_plus(W(t), S(t)), face_minus(W(t + 1), S(t + 1)), a.eos, f);
         F[0][t] = f[0];
         F[1][t] = f[1];
         F[2][t] = f[2];
@@ -1621,7 +1621,7 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
             const prim4 Rm = face_minus(W, S);
             const prim4 Rf{lane_down_z(Rm.r), lane_down_z(Rm.u), lane_down_z(Rm.p), 0.0};
             double f[4];
-            face_flux<false>(face_plus(W, S), Rf, a.eos, f);
+            face_flux<false, true>(face_plus(W, S), Rf, a.eos, f);
             u0 = u0 - dtdx * (f[0] - lane_up_z(f[0]));
             u1 = u1 - dtdx * (f[1] - lane_up_z(f[1]));
             u2 = u2 - dtdx * (f[2] - lane_up_z(f[2]));
