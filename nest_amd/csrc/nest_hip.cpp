@@ -209,6 +209,7 @@ struct nest_sim2d {
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_start = nullptr;
     std::vector<hipEvent_t> ev_rec_c, ev_rev_c;  // per interval: recompute done, reverse done
+    std::vector<hipEvent_t> ev_step;             // per reversed step: its adjoint launches done
     int roll_K = 0, roll_m = 0;
     int roll_kept = 0;  // the last roll_kept intervals' states are in their buffer sets
     double roll_dt = 0.0;
@@ -240,7 +241,7 @@ struct nest_sim2d {
             }
         if (ev_start) (void)hipEventDestroy(ev_start);
         ev_start = nullptr;
-        for (auto* v : {&ev_rec_c, &ev_rev_c}) {
+        for (auto* v : {&ev_rec_c, &ev_rev_c, &ev_step}) {
             for (hipEvent_t e : *v) (void)hipEventDestroy(e);
             v->clear();
         }
@@ -518,13 +519,18 @@ struct nest_sim2d {
         if (!ev_start) NEST_HIP_TRY(hipEventCreateWithFlags(&ev_start, hipEventDisableTiming));
         return NEST_OK;
     }
-    int ensure_interval_events(int n) {
+    int ensure_interval_events(int n, int nsteps) {
         for (auto* v : {&ev_rec_c, &ev_rev_c})
             while ((int)v->size() < n) {
                 hipEvent_t e = nullptr;
                 NEST_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 v->push_back(e);
             }
+        while ((int)ev_step.size() < nsteps) {
+            hipEvent_t e = nullptr;
+            NEST_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ev_step.push_back(e);
+        }
         return NEST_OK;
     }
     int ensure_adjoint_scratch() {
@@ -1220,9 +1226,10 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
         if (int rc = s->grow(s->seg[set], (size_t)std::max(0, m - 1))) return rc;
         if (int rc = s->grow(s->seg1[set], (size_t)m)) return rc;
     }
-    // one event per interval and direction (an event is never recorded
-    // again while a wait on its previous record may still be pending)
-    if (int rc = s->ensure_interval_events(nck)) return rc;
+    // one event per interval and direction, and one per reversed step (an
+    // event is never recorded again while a wait on its previous record may
+    // still be pending)
+    if (int rc = s->ensure_interval_events(nck, K)) return rc;
     // states of interval c: the checkpoint, then the recomputed (or kept)
     // states in buffer set c % S, with the stage-1 state of each
     auto state = [&](int c, int i) { return i == 0 ? s->ckpt[c] : s->seg[c % nsets][i - 1]; };
@@ -1230,39 +1237,65 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     // the last roll_kept intervals were kept by the rollout
     const int nkept = s->roll_kept;
     auto kept = [&](int c) { return c >= nck - nkept; };
-    auto recompute = [&](int c) -> hipError_t {
-        // interval c reuses the set that the reverse of interval c + S read
-        if (c + nsets <= nck - 1) {
-            hipError_t e = hipStreamWaitEvent(s->aux_stream, s->ev_rev_c[c + nsets], 0);
+    // step j of the recompute of interval r (auxiliary stream): its stage-1
+    // state, and the next state unless j is the interval's last step (whose
+    // successor is the next checkpoint)
+    auto recompute_step = [&](int r, int j) {
+        const int len = length(r), set = r % nsets;
+        s->stage_on(1, state(r, j), s->seg1[set][j], dt, nullptr, s->aux_stream);
+        if (j + 1 < len) s->stage_on(2, state(r, j), s->seg1[set][j], dt, state(r, j + 1), s->aux_stream);
+    };
+    // the whole recompute of interval r at once, after the reverse of
+    // interval r + S (the last to read its buffer set)
+    auto recompute = [&](int r) -> hipError_t {
+        if (r + nsets <= nck - 1) {
+            hipError_t e = hipStreamWaitEvent(s->aux_stream, s->ev_rev_c[r + nsets], 0);
             if (e != hipSuccess) return e;
         }
-        const int len = length(c), set = c % nsets;
-        for (int i = 0; i < len; ++i) {
-            s->stage_on(1, state(c, i), s->seg1[set][i], dt, nullptr, s->aux_stream);
-            if (i + 1 < len) s->stage_on(2, state(c, i), s->seg1[set][i], dt, state(c, i + 1), s->aux_stream);
-        }
-        return hipEventRecord(s->ev_rec_c[c], s->aux_stream);
+        for (int j = 0; j < length(r); ++j) recompute_step(r, j);
+        return hipEventRecord(s->ev_rec_c[r], s->aux_stream);
     };
     // everything enqueued before (the rollout and its checkpoints) comes first
     NEST_HIP_TRY(hipEventRecord(s->ev_start, s->stream));
     NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_start, 0));
+    // the intervals the reverse needs before any recompute can be spread out
     for (int c = nck - 1; c >= std::max(0, nck - (nsets - 1)); --c)
         if (!kept(c)) NEST_HIP_TRY(recompute(c));
-    // the recompute S - 1 intervals further down (auxiliary stream; it waits
-    // for the reverse of interval c + 1, the last to read its buffer set),
-    // then the reverse of interval c (handle's stream). The recompute goes
-    // into the queue first, so the auxiliary stream has it in hand when the
-    // reverse of c starts, however far ahead of the device the host is.
+    // Then, during the reverse of interval c (handle's stream), the
+    // recompute of interval r = c - (S - 1) (auxiliary stream), whose buffer
+    // set the reverse of c + 1 was the last to read: its step j waits for
+    // the j-th reversed step of c, so the auxiliary stream never holds more
+    // than one step's launches ready. (Released all at once, an interval's
+    // recompute ran alone for ~0.4 ms at each interval boundary, the reverse
+    // sweep stalled behind it.) The reverse of r, S - 1 intervals later,
+    // waits for the recompute's last step.
     bool recomputed = false;
+    int ks = 0;  // reversed steps so far (the index of their events)
     for (int c = nck - 1; c >= 0; --c) {
         const int set = c % nsets, len = length(c);
         const int r = c - (nsets - 1);
-        if (r >= 0 && !kept(r)) NEST_HIP_TRY(recompute(r));
+        // one buffer set: the interval is recomputed whole before its reverse
+        if (nsets == 1 && !kept(c)) NEST_HIP_TRY(recompute(c));
+        const bool spread = nsets > 1 && r >= 0 && !kept(r);
+        if (spread && c + 1 <= nck - 1) NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_rev_c[c + 1], 0));
         if (!kept(c)) {
             NEST_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_rec_c[c], 0));
             recomputed = true;
         }
-        for (int i = len - 1; i >= 0; --i) s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
+        const int rlen = spread ? length(r) : 0;
+        for (int i = len - 1, k = 0; i >= 0; --i, ++k) {
+            s->vjp_step_u1(state(c, i), s->seg1[set][i], Ubar, dt);
+            if (k < rlen) {
+                NEST_HIP_TRY(hipEventRecord(s->ev_step[ks], s->stream));
+                NEST_HIP_TRY(hipStreamWaitEvent(s->aux_stream, s->ev_step[ks], 0));
+                ++ks;
+                recompute_step(r, k);
+            }
+        }
+        if (spread) {
+            for (int j = len; j < rlen; ++j) recompute_step(r, j);
+            NEST_HIP_TRY(hipEventRecord(s->ev_rec_c[r], s->aux_stream));
+        }
         NEST_HIP_TRY(hipEventRecord(s->ev_rev_c[c], s->stream));
     }
     // A recompute overwrites the buffer set of a kept interval: a further

@@ -191,11 +191,13 @@ def test_stage_and_step_dot_product(oracle, case, mode):
     sim.close()
 
 
-@pytest.mark.parametrize("interval", [0, 1, 4, 16, -1])
+@pytest.mark.parametrize("interval", [0, 1, 4, 6, 16, -1])
 def test_rollout_checkpointed_adjoint(oracle, interval):
     """K = 11 fixed-dt steps with checkpoints every `interval` steps (0:
-    ceil(sqrt K) = 4; 16: one interval, recomputed whole before its reverse;
-    -1: every state kept, nothing recomputed):
+    ceil(sqrt K) = 4; 1: eleven intervals, each recomputed step by step
+    during the reverse of the interval three later; 6: two intervals, two
+    buffer sets; 16: one interval and one buffer set, recomputed whole
+    before its reverse; -1: every state kept, nothing recomputed):
     the final state is bitwise the oracle's 11 steps, and the reverse sweep
     (recompute per interval) satisfies the dot-product identity against the
     dual-number JVP of the 11-step map."""
