@@ -351,7 +351,7 @@ def adjoint_leg(mode: str, K: int = 200, ndir: int = 16, eps: float = 1e-9, reps
                              "adjoint_fwd_cost": round(t_grad_ck / t_fwd, 3),
                              "reverse_only_cost": round((t_grad_ck - t_ck) / t_fwd, 3)},
             "forward_mcups": round(cells * 2.0 * K / t_fwd / 1e6, 1),
-            "tiling": {"forward": {k: v for k, v in fwd_cfg.items() if k in ("1", "2")},
+            "tiling": {"forward": {k: v for k, v in fwd_cfg.items() if k in ("1", "2", "3", "persistent")},
                        "adjoint": {k: v for k, v in adj_cfg.items() if k != "ms_per_step"}},
             "fd_point": "initial state + seeded 1e-3 smooth perturbation of rho, u, v, p",
             "dt": float(dt).hex(),

@@ -87,15 +87,20 @@ class Rollout:
         return self._auto
 
     def tune(self, sizes, log=None):
-        """Autotune the two forward stage kernels (the rollout and its
-        recomputes run them: the reverse sweep needs every stage-1 state) and
-        the adjoint kernels' tiling on this handle (results cached per device and problem-size signature). The
+        """Autotune the forward kernels the rollouts and their recomputes run:
+        the one-launch step kernel (which also writes the stage-1 states the
+        reverse sweep reads; rollouts whose intervals are all kept, or that
+        keep a single checkpoint) and the two stage kernels (rollouts whose
+        intervals are recomputed alongside the reverse sweep); and the adjoint
+        kernels' tiling on this handle (results cached per device and
+        problem-size signature). The
         handle's state and recorded rollout are overwritten (every forward
         call reloads the state); tune before a forward, not between a forward
         and its backward."""
         from . import autotune
         self.sim.start(cfl=0.4)
         fwd = autotune.tune(self.sim, sizes, log=log, kernels="stages")
+        fwd.update(autotune.tune(self.sim, sizes, log=log, kernels="step"))
         adj = autotune.tune_adjoint(self.sim, sizes, self.dt, log=log, keep_all=self.rollout_interval() < 0)
         return fwd, adj
 

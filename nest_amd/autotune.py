@@ -160,8 +160,9 @@ def tune(sim, sizes, arch: str = "gfx950", steps: int = 8, cache: str | None = N
     """Tune `sim` (already started: start() called) in place and return the
     configuration; `sizes` = [(nx, ny)] per problem. kernels = "step": the
     kernels the handle's device-dt step runs (the one-launch step kernel
-    where it applies); "stages": the two stage kernels (what fixed-dt
-    rollouts and their recomputes run), timed through the two-launch step."""
+    where it applies; fixed-dt rollouts and their recomputes run the same
+    kernel); "stages": the two stage kernels, timed through the two-launch
+    step."""
     classes = size_classes(sizes)
     if kernels == "stages":
         sim.set_step_kernel("two_stage")

@@ -739,7 +739,9 @@ constexpr int kWaveLdsStep = kStepRing * kRowSlot;
 #define NEST_STEP_SCHED_BARRIER 1
 #endif
 
-template <int LIM>
+// U1OUT: the stage-1 state of the cells the launch updates is also written
+// to a.U1 (a rollout that the reverse sweep differentiates keeps it).
+template <int LIM, bool U1OUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) void step2d_kernel(stage_args a) {
     extern __shared__ double lds_dyn[];
     const int lane = threadIdx.x & 63;
@@ -1012,6 +1014,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
             for (int v = 0; v < 4; ++v) V[v] = d == 1 ? Q2[v] : (d == 2 ? Q1[v] : Q0[v]);
             V[1] = xor_sign(V[1], xneg);
         };
+        // U1(k) of the owned cells of row k, before the edge fixes (which
+        // change only lanes and rows outside the domain)
+        auto put_u1 = [&](int k, const double* V) {
+            if constexpr (U1OUT) {
+                if (k < r0 || k >= r1 || !store) return;
+                char* const Q8 = reinterpret_cast<char*>(a.U1);
+                const unsigned o = coff + (unsigned)(k + 2) * (unsigned)snye;
+                *reinterpret_cast<double*>(Q8 + o) = V[0];
+                *reinterpret_cast<double*>(Q8 + (o + (unsigned)svs)) = V[1];
+                *reinterpret_cast<double*>(Q8 + (o + 2u * (unsigned)svs)) = V[2];
+                *reinterpret_cast<double*>(Q8 + (o + 3u * (unsigned)svs)) = V[3];
+            } else {
+                (void)k;
+                (void)V;
+            }
+        };
         auto rotate = [&](const double* V) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -1070,6 +1088,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
             wait_lgkmcnt0();
             fetch_ahead(k);
             stage1(Uc, Ub, V);
+            put_u1(k, V);
             fix_lanes(V);
             fix_bottom(k, V);
             if (k == r0 + 1) {
@@ -1108,6 +1127,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
             wait_lgkmcnt0();
             fetch_ahead(k);
             stage1(Uc, Ub, V);
+            put_u1(k, V);
 #if NEST_STEP_SCHED_BARRIER
             __builtin_amdgcn_sched_barrier(0);
 #endif
@@ -1735,13 +1755,19 @@ void launch_stage2d(int stage, int lim, const stage_args& a, int grid, int block
     else stage2d_dispatch<2>(lim, a, grid, block, s);
 }
 
+template <bool U1OUT>
+void step2d_dispatch(int lim, const stage_args& a, int grid, int block, size_t lds, hipStream_t s) {
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((step2d_kernel<LIM_MC, U1OUT>), grid, block, lds, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((step2d_kernel<LIM_NONE, U1OUT>), grid, block, lds, s, a); break;
+        default: hipLaunchKernelGGL((step2d_kernel<LIM_MINMOD, U1OUT>), grid, block, lds, s, a); break;
+    }
+}
+// a.U1 set: the launch also writes the stage-1 state there
 void launch_step2d(int lim, const stage_args& a, int grid, int block, hipStream_t s) {
     const size_t lds = (size_t)(block / 64) * kWaveLdsStep * sizeof(double);
-    switch (lim) {
-        case LIM_MC: hipLaunchKernelGGL((step2d_kernel<LIM_MC>), grid, block, lds, s, a); break;
-        case LIM_NONE: hipLaunchKernelGGL((step2d_kernel<LIM_NONE>), grid, block, lds, s, a); break;
-        default: hipLaunchKernelGGL((step2d_kernel<LIM_MINMOD>), grid, block, lds, s, a); break;
-    }
+    if (a.U1) step2d_dispatch<true>(lim, a, grid, block, lds, s);
+    else step2d_dispatch<false>(lim, a, grid, block, lds, s);
 }
 
 int step2d_resident_blocks(int lim, int block) {
@@ -1749,9 +1775,9 @@ int step2d_resident_blocks(int lim, int block) {
     int nb = 0;
     hipError_t e;
     switch (lim) {
-        case LIM_MC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_MC>, block, lds); break;
-        case LIM_NONE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_NONE>, block, lds); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_MINMOD>, block, lds); break;
+        case LIM_MC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_MC, false>, block, lds); break;
+        case LIM_NONE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_NONE, false>, block, lds); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2d_kernel<LIM_MINMOD, false>, block, lds); break;
     }
     return e == hipSuccess ? nb : 0;
 }

@@ -210,8 +210,11 @@ struct nest_sim2d {
     hipEvent_t ev_start = nullptr;
     std::vector<hipEvent_t> ev_rec_c, ev_rev_c;  // per interval: recompute done, reverse done
     std::vector<hipEvent_t> ev_step;             // per reversed step: its adjoint launches done
+    double* roll_tmp = nullptr;  // discarded outputs of one-launch rollout steps
     int roll_K = 0, roll_m = 0;
     int roll_kept = 0;  // the last roll_kept intervals' states are in their buffer sets
+    bool roll_one = false;  // the rollout took one-launch steps
+    int roll_keep0 = 0;     // intervals the rollout kept (the last roll_keep0)
     double roll_dt = 0.0;
 
     ~nest_sim2d() { release(); }
@@ -230,8 +233,9 @@ struct nest_sim2d {
                         (void*)probs_d, (void*)slots_d, (void*)gitems_d[0], (void*)gitems_d[1], (void*)fitems_d,
                         (void*)first_item_d, (void*)geoms_d, (void*)itab_d,
                         (void*)aitems_d[0], (void*)aitems_d[1], (void*)adj_U1, (void*)adj_Vb, (void*)adj_Ub1,
-                        (void*)goff_d})
+                        (void*)goff_d, (void*)roll_tmp})
             if (p) (void)hipFree(p);
+        roll_tmp = nullptr;
         for (double* p : ckpt) (void)hipFree(p);
         ckpt.clear();
         for (int k = 0; k < NEST_ADJ_SETS; ++k)
@@ -455,13 +459,15 @@ struct nest_sim2d {
     // one-launch step from src into dst (all global strips): the caller's
     // dt (host_dt > 0), or device step nstep_dev with its dt computed in the
     // kernel from the rotating slot sets
-    void fstep(double host_dt, const double* src, double* dst) {
+    // (u1: also the step's stage-1 state of every interior cell; on: a
+    // stream other than the handle's)
+    void fstep(double host_dt, const double* src, double* dst, double* u1 = nullptr, hipStream_t on = nullptr) {
         const int n = (int)fitems_h.size();
         stage_args a = args(host_dt, 0, n, 0);
         a.items = fitems_d;
         a.gather = 1;
         a.U = const_cast<double*>(src);
-        a.U1 = nullptr;
+        a.U1 = u1;
         a.Uo = dst;
         if (!(host_dt > 0.0)) {
             a.slots_rd = slot_set(nstep_dev);
@@ -470,7 +476,7 @@ struct nest_sim2d {
             a.first_item = first_item_d;
             a.par = nstep_dev & 1;
         }
-        L().step2d(lim, a, grid_for(n, fwpb, fresident), 64 * fwpb, stream);
+        L().step2d(lim, a, grid_for(n, fwpb, fresident), 64 * fwpb, on ? on : stream);
     }
     // dt of the two-launch step: fold the read set, clear the set the step
     // after next writes, schedule the step
@@ -1186,6 +1192,21 @@ int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
     // every later one is written by the stage 2 before it (out of place) and
     // read from there by the next step
     NEST_HIP_TRY(hipMemcpyAsync(s->ckpt[0], s->U, s->total * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    // One launch per step (out of place) where the step kernel applies; it
+    // also writes the stage-1 state, kept for the reverse sweep or (interval
+    // not kept) into a discarded buffer.
+    // The reverse sweep's recompute of a single interval runs the same
+    // kernel instance, so it reproduces the forward's states bit for bit.
+    // Intervals recomputed alongside the reverse of later ones run the two
+    // stage kernels instead, which share the GPU with the adjoint kernels
+    // better (the step kernel's 227 VGPRs and 16 KB of LDS per wave crowd
+    // them out: at 1024^2, K = 200 in 15-step intervals the gradient took
+    // 41.2 against 40.7 ms). Their fast arithmetic rounds differently from
+    // the step kernel's, so those intervals are reversed about states within
+    // round-off of the forward's (exact numerics: bitwise the same).
+    const bool one = s->one_launch();
+    if (one && !s->roll_tmp)
+        if (int rc = dev_alloc(&s->roll_tmp, s->total)) return rc;
     for (int c = 0, k = 0; c < nck; ++c) {
         const int len = interval_start(nsteps, m, nck, c + 1) - interval_start(nsteps, m, nck, c);
         const bool kept = c >= nck - nkeep;
@@ -1196,13 +1217,28 @@ int nest_hip_sim2d_rollout(nest_sim2d* s, int nsteps, double dt, int interval) {
             double* next = i + 1 < len ? (kept ? s->seg[set][i] : s->U) : (c + 1 < nck ? s->ckpt[c + 1] : s->U);
             double* src = i == 0 ? (k > 0 ? s->ckpt[c] : s->U) : (kept ? s->seg[set][i - 1] : s->U);
             double* u1 = kept ? s->seg1[set][i] : s->U1;
-            s->stage_on(1, src, u1, dt);
-            s->stage_on(2, src, u1, dt, next == src ? nullptr : next);
+            if (one) {
+                // every interval starts from its checkpoint (ckpt[0]: the
+                // copy of U); the intermediate states of an interval that is
+                // not kept alternate between U1 and U, phased so that its
+                // last step reads U1: state j goes to U1 when len - 2 - j is
+                // even
+                auto inter = [&](int j) { return ((len - 2 - j) & 1) ? s->U : s->U1; };
+                if (i == 0) src = s->ckpt[c];
+                else if (!kept) src = inter(i - 1);
+                if (!kept && i + 1 < len) next = inter(i);
+                s->fstep(dt, src, next, kept ? u1 : s->roll_tmp);
+            } else {
+                s->stage_on(1, src, u1, dt);
+                s->stage_on(2, src, u1, dt, next == src ? nullptr : next);
+            }
         }
     }
     s->fill(s->U);
     s->speeds_stale = true;
     s->roll_kept = nkeep;
+    s->roll_one = one;
+    s->roll_keep0 = nkeep;
     s->roll_K = nsteps;
     s->roll_m = m;
     s->roll_dt = dt;
@@ -1237,11 +1273,24 @@ int nest_hip_sim2d_rollout_adjoint(nest_sim2d* s, double* Ubar) {
     // the last roll_kept intervals were kept by the rollout
     const int nkept = s->roll_kept;
     auto kept = [&](int c) { return c >= nck - nkept; };
+    // One-launch steps (the output of the interval's last step discarded)
+    // where the forward took them and the recompute must reproduce its states
+    // bit for bit: a single interval, and the intervals the forward kept (a
+    // second reverse of the same rollout recomputes them after the first
+    // reverse's recomputes overwrote their buffer sets). The two stage
+    // kernels for the other intervals, which the forward did not keep.
+    auto one_at = [&](int r) { return s->roll_one && (nck == 1 || r >= nck - s->roll_keep0); };
+    if (s->roll_one && !s->roll_tmp)
+        if (int rc = dev_alloc(&s->roll_tmp, s->total)) return rc;
     // step j of the recompute of interval r (auxiliary stream): its stage-1
     // state, and the next state unless j is the interval's last step (whose
     // successor is the next checkpoint)
     auto recompute_step = [&](int r, int j) {
         const int len = length(r), set = r % nsets;
+        if (one_at(r)) {
+            s->fstep(dt, state(r, j), j + 1 < len ? state(r, j + 1) : s->roll_tmp, s->seg1[set][j], s->aux_stream);
+            return;
+        }
         s->stage_on(1, state(r, j), s->seg1[set][j], dt, nullptr, s->aux_stream);
         if (j + 1 < len) s->stage_on(2, state(r, j), s->seg1[set][j], dt, state(r, j + 1), s->aux_stream);
     };

@@ -103,23 +103,35 @@ def _dual_check(oracle, U0, grad, target, n, npx, npy, K, dt, ndir, seed):
     return worst
 
 
-def test_gradient_independent_of_state_storage():
-    """The same gradient, bitwise, whether the rollout keeps every state
-    (interval -1, and the automatic choice at this size), checkpoints every 5
-    or 2 steps (3 intervals, all kept; 6 intervals, the first two
-    recomputed), or keeps only the initial state (interval >= K: the whole
-    rollout recomputed); and the automatic choice keeps every state when it
-    fits."""
-    print_case("gradient vs state storage 96^2 K=12")
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_gradient_independent_of_state_storage(mode):
+    """The same gradient whether the rollout keeps every state (interval -1,
+    and the automatic choice at this size), checkpoints every 5 or 2 steps
+    (3 intervals, all kept; 6 intervals, the first two recomputed alongside
+    the reverse sweep), or keeps only the initial state (interval >= K: the
+    whole rollout recomputed); and the automatic choice keeps every state
+    when it fits. Bitwise in exact numerics. In fast numerics bitwise where
+    the forward runs the same kernels: the one-launch step wherever no
+    interval is recomputed alongside the reverse sweep (-1, 5, 12), the two
+    stage kernels where one is (2: within 1e-12 of the others, the two step
+    forms' fast arithmetic rounding differently)."""
+    print_case("gradient vs state storage 96^2 K=12", mode)
     grads = {}
     for interval in (-1, 0, 5, 2, 12):
-        ro, U0, U0d, loss, grad, target, dt = _setup(96, 12, "fast", npx=2, npy=3, interval=interval)
+        ro, U0, U0d, loss, grad, target, dt = _setup(96, 12, mode, npx=2, npy=3, interval=interval)
         grads[interval] = grad.detach().cpu().numpy()
         if interval == 0:
             assert ro.rollout_interval() == -1
         ro.close()
-    for interval in (0, 5, 2, 12):
+    for interval in (0, 5, 12):
         assert np.array_equal(grads[interval], grads[-1]), interval
+    if mode == "exact":
+        assert np.array_equal(grads[2], grads[-1])
+    else:
+        g, r = grads[2], grads[-1]
+        err = np.max(np.abs(g - r)) / np.max(np.abs(r))
+        print(f"two-stage against one-launch forward: {err:.2e}")
+        assert err <= 1e-12
 
 
 def test_torch_gradient_small(oracle):
