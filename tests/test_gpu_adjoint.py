@@ -201,10 +201,23 @@ def test_rollout_checkpointed_adjoint(oracle, interval):
     the final state is bitwise the oracle's 11 steps, and the reverse sweep
     (recompute per interval) satisfies the dot-product identity against the
     dual-number JVP of the 11-step map."""
-    nx, ny, npx, npy, K = 64, 48, 2, 2, 11
-    print_case("adjoint rollout", interval)
-    m = MeshSpec(nx, ny, npx, npy, bcx="periodic", bcy="periodic")
-    sim = Sim2D.mesh(nx, ny, npx, npy, bcx="periodic", bcy="periodic", mode="exact")
+    _rollout_check(oracle, interval, 64, 48, 2, 2, "periodic", "periodic")
+
+
+@pytest.mark.parametrize("interval", [4, -1])
+@pytest.mark.parametrize("bcs", [("outflow", "outflow"), ("outflow", "periodic")])
+def test_rollout_checkpointed_adjoint_outflow(oracle, interval, bcs):
+    """The same on an uneven 3 x 2 mesh with outflow edges: the one-launch
+    forward's stage-1 states at the edges (copied from the boundary cells for
+    the second stage) are the ones the reverse reads."""
+    _rollout_check(oracle, interval, 70, 53, 3, 2, *bcs)
+
+
+def _rollout_check(oracle, interval, nx, ny, npx, npy, bcx, bcy):
+    K = 11
+    print_case("adjoint rollout", interval, nx, ny, npx, npy, bcx, bcy)
+    m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+    sim = Sim2D.mesh(nx, ny, npx, npy, bcx=bcx, bcy=bcy, mode="exact")
     dx, dy = 1.0 / nx, 1.0 / ny
     dt = 0.3 * min(dx, dy) / 1.5
     U = sim.scatter(rough_state(nx, ny, seed=21, vmax=0.4))
