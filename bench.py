@@ -171,9 +171,15 @@ def cpu_model() -> str:
 def ensemble_leg(steps: int, mode: str):
     """Ensemble: 512 uneven four-quadrant problems, 100 RK2 steps with
     per-problem device dt. Batched (one launch per stage over all problems,
-    hipGraph-replayed step, autotuned tiles) against the plain per-patch path
-    (host dt with a blocking read-back, one launch set per problem)."""
-    from nest_amd import autotune, problems as P
+    hipGraph-replayed step, autotuned tiles) against two per-patch paths:
+    the plain one (host dt with a blocking read-back, one launch set per
+    problem), and one handle per problem with its dt on the device (two
+    steps per call, no host round trip), every handle on one stream and then
+    each on a stream of its own."""
+    import numpy as np
+    import torch
+
+    from nest_amd import Sim2D, autotune, problems as P
 
     sim, sizes, U0 = P.ensemble_batch(512, seed=3, mode=mode)
     sim_one = sim.one_launch
@@ -196,12 +202,52 @@ def ensemble_leg(steps: int, mode: str):
     sim.advance_per_patch(pp_steps, 0.4)
     sim.synchronize()
     tp = time.perf_counter() - t0
+    states = [np.ascontiguousarray(sim.patch_view(U0, k)).copy() for k in range(len(sizes))]
     sim.close()
+    # one handle per problem (its single patch), device dt
+    st = torch.cuda.Stream()  # default priority
+    hs = []
+    for (s, _), Uk in zip(sizes, states):
+        h = Sim2D.mesh(s, s, 1, 1, mode=mode)
+        h.set_stream(st.cuda_stream)
+        h.upload(Uk)
+        h.start(cfl=0.4)
+        hs.append(h)
+    for h in hs:
+        h.advance(2)
+    st.synchronize()
+    dd_steps = 2 * max(3, steps // 20)
+    t0 = time.perf_counter()
+    for _ in range(dd_steps // 2):
+        for h in hs:
+            h.advance(2)
+    st.synchronize()
+    td = time.perf_counter() - t0
+    # the same with every handle on a stream of its own (concurrent launches)
+    for h in hs:
+        h.set_stream(None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(dd_steps // 2):
+        for h in hs:
+            h.advance(2)
+    for h in hs:
+        h.synchronize()
+    tc = time.perf_counter() - t0
+    for h in hs:
+        h.close()
     batched = cells * 2.0 * steps / tb / 1e6
     per_patch = cells * 2.0 * pp_steps / tp / 1e6
+    per_problem = cells * 2.0 * dd_steps / td / 1e6
+    per_problem_streams = cells * 2.0 * dd_steps / tc / 1e6
     return {"problems": 512, "cells": int(cells), "steps": steps, "batched_mcups": round(batched, 1),
             "per_patch_mcups": round(per_patch, 1), "speedup": round(batched / per_patch, 2),
             "per_patch_steps_timed": pp_steps,
+            "per_problem_device_dt_mcups": round(per_problem, 1),
+            "speedup_vs_per_problem_device_dt": round(batched / per_problem, 2),
+            "per_problem_device_dt_steps_timed": dd_steps,
+            "per_problem_device_dt_own_streams_mcups": round(per_problem_streams, 1),
+            "speedup_vs_per_problem_own_streams": round(batched / per_problem_streams, 2),
             "step_kernel": "one launch per step" if sim_one else "two stage launches",
             "tiling": {k: v for k, v in cfg.items() if k in ("1", "2", "3", "persistent")}}
 
