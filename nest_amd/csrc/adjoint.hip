@@ -114,6 +114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         const unsigned col = (unsigned)(qd.off + lc);
         const bool ring = G.bcy == 0;
         const bool real = ring || (gcol >= 0 && gcol < G.NY);  // outflow ghosts are not outputs
+        const double greal = real ? a.g : 0.0;                   // the cotangent's scale on this lane
         const bool own = lane >= 3 && lane < 61 && gcol < G.NY;
         const bool edge = !ring && (gs == 0 || gs + 58 >= G.NY);  // wave-uniform
 
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEST_ADJY_W
         // the stage-output cotangent A (unscaled); writes Vb(i)
         auto yrev = [&](int i, const double* v, double* A) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) A[k] = real ? a.g * A[k] : 0.0;
+            for (int k = 0; k < 4; ++k) A[k] = greal * A[k];
             const prim4 Y = prim2d(v[0], v[1], v[2], v[3], false, a.eos);
             const prim4 dl = sub4(Y, up1(Y));
             const prim4 dr = dn1(dl);

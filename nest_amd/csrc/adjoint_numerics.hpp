@@ -208,8 +208,8 @@ __device__ __forceinline__ void face_flux_rev_clamped(const prim4& L, const prim
     double rRb = R.u * (mRb + kRb * R.u) + du0;
     double pRb = fr1 + g1 * hRb - du2;
     // Davis speeds and the sound speeds under them
-    const double aLb = sl_from_r ? 0.0 : SLb, aRb = sl_from_r ? SLb : 0.0;
-    const double bLb = sr_from_r ? 0.0 : SRb, bRb = sr_from_r ? SRb : 0.0;
+    const double aLb = sl_from_r ? 0.0 : SLb, aRb = SLb - aLb;
+    const double bLb = sr_from_r ? 0.0 : SRb, bRb = SRb - bLb;
     uLb += aLb + bLb;
     uRb += aRb + bRb;
     const double cLb = bLb - aLb, cRb = bRb - aRb;
