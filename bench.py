@@ -458,7 +458,7 @@ def oned_leg(large_steps: int = 200, sod_t_end: float = 0.2):
 
 def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
     """Kelvin-Helmholtz 8192^2 periodic on 64x64 patches of 128^2,
-    `steps` RK2 steps with device dt (graph-replayed); then `nadj`
+    `steps` RK2 steps with device dt (graph-replayed, autotuned tiling); then `nadj`
     concurrent adjoint gradients of 256^2 instances of the adjoint leg's setup
     (each its own seed) in one batch handle, K = adj_steps, against the same
     gradients one problem at a time; then both together (the KH steps and the
@@ -481,6 +481,9 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
         sim.synchronize()
 
     kh_reset()
+    from nest_amd import autotune
+    kh_cfg = autotune.tune(sim, [(n, n)], log=log)
+    kh_reset()
     sim.advance(2, graph=True)
     kh_reset()
     t0 = time.perf_counter()
@@ -490,7 +493,8 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
     st = sim.status()
     kh = {"cells": n * n, "patches": npch * npch, "steps": steps,
           "mcups": round(n * n * 2.0 * steps / t_kh / 1e6, 1), "ms_per_step": round(t_kh * 1e3 / steps, 3),
-          "t_reached": float(f"{st['t'][0]:.6g}")}
+          "t_reached": float(f"{st['t'][0]:.6g}"),
+          "tiling": {k: v for k, v in kh_cfg.items() if k in ("1", "2", "3", "persistent")}}
 
     m = 256
     setups = [P.adjoint_setup(m, seed=10 + 3 * b) for b in range(nadj)]
