@@ -168,6 +168,16 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def lane_efficiency(sizes, width: int = 56) -> float:
+    """Share of the step kernel's lane-rows that produce an output cell: a
+    problem of nx x ny cells runs ceil(ny / width) 64-lane strips down its nx
+    rows (the one-launch step kernel's strips produce 56 columns, the
+    two-launch step's 60); warm-up rows not counted."""
+    out = sum(a * b for a, b in sizes)
+    lanes = sum(64 * (-(-b // width)) * a for a, b in sizes)
+    return out / lanes
+
+
 def ensemble_leg(steps: int, mode: str):
     """Ensemble: 512 uneven four-quadrant problems, 100 RK2 steps with
     per-problem device dt. Batched (one launch per stage over all problems,
@@ -249,6 +259,7 @@ def ensemble_leg(steps: int, mode: str):
             "per_problem_device_dt_own_streams_mcups": round(per_problem_streams, 1),
             "speedup_vs_per_problem_own_streams": round(batched / per_problem_streams, 2),
             "step_kernel": "one launch per step" if sim_one else "two stage launches",
+            "lane_efficiency": round(lane_efficiency(sizes, 56 if sim_one else 60), 4),
             "tiling": {k: v for k, v in cfg.items() if k in ("1", "2", "3", "persistent")}}
 
 
@@ -787,6 +798,7 @@ def main():
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "kernels_hbm_frac": round(stages_gbs / HBM_PEAK_GBS, 4),
         "step_kernel": "one launch per step (U1 on chip)" if one else "two stage launches",
+        "lane_efficiency": round(lane_efficiency([(N, N)], 56 if one else 60), 4),
         "kernel_ms": ({"step": round(k1_ms, 4)} if one else
                       {"finalize": round(comp_ms[0] / args.steps, 4), "stage1": round(k1_ms, 4),
                        "stage2": round(k2_ms, 4)}),
