@@ -1539,6 +1539,59 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
     }
 }
 
+// The device-dt step's dt (finalize_kernel's arithmetic) from the speed
+// slots the previous step wrote, the same in every wave. Wave 0 records the
+// step (dt, time, step count, or the done flag) and clears the slot set of
+// the next step. Returns 0 (take the step), 1 / 2 (the run ends here: end
+// time reached / no usable dt) or -1 (the run had already ended).
+__device__ __forceinline__ int step1d_schedule(const stage1d_args& a, int wid, int lane, double& dt) {
+    const int done_in = a.run->done;
+    const double t_in = a.run->t[a.par];
+    const unsigned long long v_in = lane < kStepSlots ? a.slots_rd[(size_t)lane * NEST_SLOT_STRIDE] : 0ull;
+    const unsigned long long v = wave_max_u64(v_in);
+    const nest_prob_state& P = *a.ps;
+    const double sx = __longlong_as_double((long long)v);
+    int done = 0;
+    dt = 0.0;
+    if (done_in) {
+        done = -1;  // already finished: leave the run's state alone
+    } else if (!(t_in < P.t_end)) {
+        done = 1;
+    } else {
+        dt = P.fixed_dt > 0.0 ? P.fixed_dt : P.cfl * P.dx / sx;
+        dt = dmin(dt, P.t_end - t_in);
+        if (!dt_ok(dt) || bits_nan(sx)) done = 2;
+    }
+    if (wid == 0) {
+        if (lane == 0 && done >= 0) {
+            if (done) {
+                a.run->done = done;
+                if (done == 2) a.run->dt = dt;
+            } else {
+                a.run->dt = dt;
+                a.run->t[a.par ^ 1] = t_in + dt;
+                a.run->steps += 1;
+            }
+        }
+        if (lane < kStepSlots) a.slots_clr[(size_t)lane * NEST_SLOT_STRIDE] = 0ull;  // for the next step
+    }
+    return done;
+}
+
+// the wave's speed maximum meets the block's in LDS and reaches the slots
+// with one atomic per block
+__device__ __forceinline__ void step1d_fold_speeds(const stage1d_args& a, unsigned long long sm,
+                                                   unsigned long long* wmax) {
+    sm = wave_max_u64(sm);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = b > wmax[w] ? b : wmax[w];
+        atomicMax(a.slots + (size_t)(blockIdx.x & (kStepSlots - 1)) * NEST_SLOT_STRIDE, b);
+    }
+}
+
 // The same whole 1D RK2 step with no LDS tile and no barrier inside the
 // step: every wave works on its own. Lane l of a segment holds cell
 // g0 - 4 + l; neighbour values move by DPP wave shifts (the Y-sweep
@@ -1547,8 +1600,7 @@ __global__ __launch_bounds__(256) void step1d_kernel(stage1d_args a) {
 // outflow / reflecting wall, U1 comes from the lane of the mapped cell
 // (nest_step_lane_source), momentum negated at a reflecting wall; past a
 // periodic end stage 1 sees the stencil of the cell it wraps to. Each wave
-// runs a.segs consecutive segments (one dt read per wave); its speed maximum
-// meets the block's in LDS and reaches the slots with one atomic per block.
+// runs a.segs consecutive segments (one dt read per wave).
 // Arithmetic and order per cell are those of step1d_kernel.
 template <int LIM>
 __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
@@ -1572,42 +1624,11 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
         }
     };
     // the first segment's loads go out before the dt arithmetic, which they
-    // do not depend on; then the run's flag, this step's time, the previous
-    // step's speeds
+    // do not depend on
     fetch(0);
-    const int done_in = a.run->done;
-    const double t_in = a.run->t[a.par];
-    const unsigned long long v_in = lane < kStepSlots ? a.slots_rd[(size_t)lane * NEST_SLOT_STRIDE] : 0ull;
-    const unsigned long long v = wave_max_u64(v_in);
-    // the step's dt (schedule_step's arithmetic), the same in every wave
-    const nest_prob_state& P = *a.ps;
-    const double sx = __longlong_as_double((long long)v);
-    int done = 0;
-    double dt = 0.0;
-    if (done_in) {
-        done = -1;  // already finished: leave the run's state alone
-    } else if (!(t_in < P.t_end)) {
-        done = 1;
-    } else {
-        dt = P.fixed_dt > 0.0 ? P.fixed_dt : P.cfl * P.dx / sx;
-        dt = dmin(dt, P.t_end - t_in);
-        if (!dt_ok(dt) || bits_nan(sx)) done = 2;
-    }
-    if (wid == 0) {
-        if (lane == 0 && done >= 0) {
-            if (done) {
-                a.run->done = done;
-                if (done == 2) a.run->dt = dt;
-            } else {
-                a.run->dt = dt;
-                a.run->t[a.par ^ 1] = t_in + dt;
-                a.run->steps += 1;
-            }
-        }
-        if (lane < kStepSlots) a.slots_clr[(size_t)lane * NEST_SLOT_STRIDE] = 0ull;  // for the next step
-    }
-    const bool go = !done;
-    const double dtdx = dt / P.dx;
+    double dt;
+    const bool go = step1d_schedule(a, wid, lane, dt) == 0;
+    const double dtdx = dt / a.ps->dx;
     unsigned long long sm = 0;
     for (int sg = 0; sg < a.segs; ++sg) {
         const int g0 = (wid * a.segs + sg) * 56;
@@ -1669,14 +1690,182 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
         }
     }
     if (!go) return;
-    sm = wave_max_u64(sm);
-    if (lane == 0) wmax[wib] = sm;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long b = wmax[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = b > wmax[w] ? b : wmax[w];
-        atomicMax(a.slots + (size_t)(blockIdx.x & (kStepSlots - 1)) * NEST_SLOT_STRIDE, b);
+    step1d_fold_speeds(a, sm, wmax);
+}
+
+// The lane step with two cells per lane: lane l of a segment holds cells
+// g0 - 4 + 2l (A) and g0 - 3 + 2l (B). A cell's left neighbour is the B
+// cell of the lane before (A) or the lane's own A cell (B), so each stage
+// moves one value per quantity between lanes where the one-cell kernel
+// moves two, and the face between A and B is computed on the lane. Stage 1
+// is valid on lanes 1..62, stage 2 on lanes 2..61: a segment of 128 cells
+// produces 120. Past an outflow / reflecting wall, U1 comes from the lane
+// and half of the mapped cell, momentum negated at a reflecting wall; past
+// a periodic end stage 1 sees the stencil of the cell it wraps to. Each
+// wave runs a.segs consecutive segments. WIDE (state buffers and the
+// variable stride 16-byte aligned: n even): a lane moves its two cells with
+// one 16-byte access per variable (a wave's access spans whole cache lines,
+// where two 8-byte accesses each touch every other 8 bytes of twice as
+// many), and segments with a cell past the domain load cell by cell through
+// the boundary map. Arithmetic and order per cell are those of
+// step1d_kernel.
+typedef double nest_d2 __attribute__((ext_vector_type(2)));
+template <int LIM, bool WIDE>
+__global__ __launch_bounds__(256) void step1d_pairs_kernel(stage1d_args a) {
+    __shared__ unsigned long long wmax[4];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wib);
+    const int n = a.n;
+    // byte offsets from the state buffers (3 (n + 4) doubles: below 4 GiB
+    // wherever this kernel is chosen)
+    const char* const in8 = reinterpret_cast<const char*>(a.U);
+    char* const out8 = reinterpret_cast<char*>(a.U1);
+    const unsigned vs = (unsigned)(n + 4) * 8u;
+    auto ld = [&](unsigned o) { return *reinterpret_cast<const double*>(in8 + o); };
+    auto st = [&](unsigned o, double x) { *reinterpret_cast<double*>(out8 + o) = x; };
+    // U^n of segment sg's cells A and B (through the boundary map; momentum
+    // not yet negated past a reflecting wall)
+    double pA[3] = {0.0, 0.0, 0.0}, pB[3] = {0.0, 0.0, 0.0};
+    auto fetch = [&](int sg) {
+        const int g0 = (wid * a.segs + sg) * 120;
+        if (sg < a.segs && g0 < n) {
+            const int gA = g0 - 4 + 2 * lane;
+            if (WIDE && g0 >= 4 && g0 + 124 <= n) {
+                const unsigned oA = (unsigned)(gA + 2) * 8u;
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const nest_d2 x = *reinterpret_cast<const nest_d2*>(in8 + (oA + (unsigned)v * vs));
+                    pA[v] = x.x;
+                    pB[v] = x.y;
+                }
+                return;
+            }
+            const unsigned oA = (unsigned)(nest_map_index(gA, n, a.bc) + 2) * 8u;
+            const unsigned oB = (unsigned)(nest_map_index(gA + 1, n, a.bc) + 2) * 8u;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                pA[v] = ld(oA + (unsigned)v * vs);
+                pB[v] = ld(oB + (unsigned)v * vs);
+            }
+        }
+    };
+    fetch(0);
+    double dt;
+    const bool go = step1d_schedule(a, wid, lane, dt) == 0;
+    const double dtdx = dt / a.ps->dx;
+    unsigned long long sm = 0;
+    for (int sg = 0; sg < a.segs; ++sg) {
+        const int g0 = (wid * a.segs + sg) * 120;
+        if (g0 >= n) break;
+        const int gA = g0 - 4 + 2 * lane, gB = gA + 1;
+        const bool mid = lane >= 2 && lane < 62;
+        const bool storeA = mid && gA < n, storeB = mid && gB < n;
+        double A[3], B[3];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            A[v] = pA[v];
+            B[v] = pB[v];
+        }
+        fetch(sg + 1);  // the next segment's loads, in flight during this one
+        const unsigned eA = (unsigned)(gA + 2) * 8u;  // cell A's byte offset (stored lanes)
+        // both cells of a lane (n even: storeA == storeB)
+        auto st2 = [&](unsigned o, double x, double y) {
+            if constexpr (WIDE) {
+                nest_d2 q;
+                q.x = x;
+                q.y = y;
+                *reinterpret_cast<nest_d2*>(out8 + o) = q;
+            } else {
+                if (storeA) st(o, x);
+                if (storeB) st(o + 8u, y);
+            }
+        };
+        if (!go) {
+            // a step past the end passes the state through
+            if (storeA || storeB) {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) st2(eA + (unsigned)v * vs, A[v], B[v]);
+            }
+            continue;
+        }
+        const bool outA = gA < 0 || gA >= n, outB = gB < 0 || gB >= n;
+        if (a.bc == 2) {
+            if (outA) A[1] = -A[1];
+            if (outB) B[1] = -B[1];
+        }
+        const double zA[3] = {A[0], A[1], A[2]}, zB[3] = {B[0], B[1], B[2]};  // U^n (the combine)
+        // one stage: c <- c - dtdx (F(c+1/2) - F(c-1/2)) on both cells
+        auto stage = [&](double* a_, double* b_) {
+            const prim4 WA = prim1d(a_[0], a_[1], a_[2], a.eos);
+            const prim4 WB = prim1d(b_[0], b_[1], b_[2], a.eos);
+            // left differences: A - (B of the lane before), B - A
+            const prim4 dA{WA.r - lane_up_z(WB.r), WA.u - lane_up_z(WB.u), WA.p - lane_up_z(WB.p), 0.0};
+            const prim4 dB{WB.r - WA.r, WB.u - WA.u, WB.p - WA.p, 0.0};
+            const prim4 SA = slope4<LIM>(dA, dB);
+            const prim4 SB = slope4<LIM>(dB, prim4{lane_down_z(dA.r), lane_down_z(dA.u), lane_down_z(dA.p), 0.0});
+            const prim4 RA = face_minus(WA, SA);
+            double fA[4], fB[4];
+            face_flux<false, true>(face_plus(WA, SA), face_minus(WB, SB), a.eos, fA);  // face A | B
+            face_flux<false, true>(face_plus(WB, SB), prim4{lane_down_z(RA.r), lane_down_z(RA.u), lane_down_z(RA.p), 0.0},
+                                   a.eos, fB);  // face B | A of the lane after
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                a_[v] = a_[v] - dtdx * (fA[v] - lane_up_z(fB[v]));
+                b_[v] = b_[v] - dtdx * (fB[v] - fA[v]);
+            }
+        };
+        stage(A, B);
+        if (a.bc != 0 && (g0 - 4 < 0 || g0 + 123 >= n)) {
+            // U1 past an outflow / reflecting wall from the lane and half of
+            // the mapped cell (clamped into the segment for cells far past
+            // the wall, whose values stage 2 never reads)
+            auto pos = [&](int g, int own) {
+                if (g >= 0 && g < n) return own;
+                const int q = nest_map_index(g, n, a.bc) - (g0 - 4);
+                return q < 0 ? 0 : (q > 127 ? 127 : q);
+            };
+            const int qA = pos(gA, 2 * lane), qB = pos(gB, 2 * lane + 1);
+            double nA[3], nB[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const double xa = __shfl(A[v], qA >> 1), xb = __shfl(B[v], qA >> 1);
+                const double ya = __shfl(A[v], qB >> 1), yb = __shfl(B[v], qB >> 1);
+                nA[v] = (qA & 1) ? xb : xa;
+                nB[v] = (qB & 1) ? yb : ya;
+            }
+            if (a.bc == 2) {
+                if (outA) nA[1] = -nA[1];
+                if (outB) nB[1] = -nB[1];
+            }
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                A[v] = nA[v];
+                B[v] = nB[v];
+            }
+        }
+        stage(A, B);
+        if (storeA || storeB) {
+            double oA[3], oB[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                oA[v] = 0.5 * (zA[v] + A[v]);
+                oB[v] = 0.5 * (zB[v] + B[v]);
+            }
+            auto speed = [&](const double* o, bool own) {
+                const prim4 w = prim1d(o[0], o[1], o[2], a.eos);
+                const double sp = __builtin_fabs(w.u) + sound_speed_1d(w.r, w.p, a.eos);
+                const unsigned long long b = speed_slot_bits(sp, o[0], o[2]);
+                sm = own && b > sm ? b : sm;
+            };
+            speed(oA, storeA);
+            speed(oB, storeB);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) st2(eA + (unsigned)v * vs, oA[v], oB[v]);
+        }
     }
+    if (!go) return;
+    step1d_fold_speeds(a, sm, wmax);
 }
 
 __global__ __launch_bounds__(256) void speeds1d_kernel(stage1d_args a) {
@@ -1817,7 +2006,24 @@ void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s) {
     else stage1d_dispatch<2>(lim, a, grid, s);
 }
 
+template <bool WIDE>
+static void pairs_dispatch(int lim, const stage1d_args& a, int grid, hipStream_t s) {
+    switch (lim) {
+        case LIM_MC: hipLaunchKernelGGL((step1d_pairs_kernel<LIM_MC, WIDE>), grid, 256, 0, s, a); break;
+        case LIM_NONE: hipLaunchKernelGGL((step1d_pairs_kernel<LIM_NONE, WIDE>), grid, 256, 0, s, a); break;
+        default: hipLaunchKernelGGL((step1d_pairs_kernel<LIM_MINMOD, WIDE>), grid, 256, 0, s, a); break;
+    }
+}
+
 void launch_step1d(int lim, const stage1d_args& a, hipStream_t s) {
+    if (a.segs > 0 && a.pairs) {  // two cells per lane: a.segs segments of 120 cells per wave
+        const int waves = (a.n + 120 * a.segs - 1) / (120 * a.segs);
+        const int grid = (waves + 3) / 4;
+        const bool wide = (((uintptr_t)a.U | (uintptr_t)a.U1) & 15) == 0 && (a.n & 1) == 0;
+        if (wide) pairs_dispatch<true>(lim, a, grid, s);
+        else pairs_dispatch<false>(lim, a, grid, s);
+        return;
+    }
     if (a.segs > 0) {  // lane kernel: a.segs segments of 56 cells per wave, 4 waves per block
         const int waves = (a.n + 56 * a.segs - 1) / (56 * a.segs);
         const int grid = (waves + 3) / 4;

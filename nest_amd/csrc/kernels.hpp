@@ -101,8 +101,10 @@ struct stage1d_args {
     unsigned long long* slots_clr;
     int par;
     // fused step kernel: 0 the LDS-tile kernel (256 cells per block); > 0
-    // the lane kernel with this many 56-cell segments per wave
+    // a lane kernel with this many segments per wave: of 56 cells (one cell
+    // per lane), or of 120 cells when pairs is set (two cells per lane)
     int segs;
+    int pairs;
 };
 
 NEST_DECLARE_LAUNCHERS(nest_exact)

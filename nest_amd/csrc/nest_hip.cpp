@@ -1368,16 +1368,29 @@ int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval,
 // ===========================================================================
 // 1D solver
 // ===========================================================================
-// The fused 1D step kernel for n cells: 0 = the LDS-tile kernel, > 0 = the
-// lane kernel with that many 56-cell segments per wave (about 16k waves per
-// launch, so each wave's dt read and the block atomics are amortised).
-// kind: 0 automatic (the lane kernel: measured 7.1 against 10.3 us per step
-// at n = 1e5, 262 against 406 us at n = 2^24), 1 LDS tile, 2 lanes.
-static int step1d_segs(int n, int kind) {
-    const bool lanes = kind != 1;
-    if (!lanes) return 0;
-    const long long per = 56LL * 16384;
-    return (int)std::max(1LL, ((long long)n + per - 1) / per);
+// The pair kernel addresses the state (3 (n + 4) doubles) with 32-bit byte
+// offsets.
+static bool step1d_pairs_fit(int n) { return 24ULL * ((unsigned long long)n + 4) < (1ULL << 32); }
+// The fused 1D step kernel for n cells: kind 1 the LDS-tile kernel (segs =
+// 0); kind 2 the lane kernel (one cell per lane, 56-cell segments), kind 3
+// the pair kernel (two cells per lane, 120-cell segments), each wave running
+// segs segments (about 16k waves per launch, so each wave's dt read and the
+// block atomics are amortised). Kind 0 automatic: the pair kernel from 2^19
+// cells (where it fits), the lane kernel below. Measured per step (fast /
+// exact numerics): 2^24 cells 186.9 / 322.4 us against 204.6 / 350.8 us for
+// the lane kernel, 2^19 12.0 / 17.1 against 13.1 / 18.4, 2^18 8.5 / 11.7
+// against 8.6 / 11.3, 1e5 6.2 / 6.9 against 6.3 / 6.8, 1000 5.8 / 6.7
+// against 5.3 / 5.8 (the lane kernel: 7.1 against 10.3 us at 1e5 and 262
+// against 406 us at 2^24 for the tile kernel).
+static void step1d_kernel_args(int n, int kind, stage1d_args& a) {
+    if (kind == 0) kind = n >= (1 << 19) && step1d_pairs_fit(n) ? 3 : 2;
+    a.pairs = kind == 3;
+    if (kind == 1) {
+        a.segs = 0;
+        return;
+    }
+    const long long per = (a.pairs ? 120LL : 56LL) * 16384;
+    a.segs = (int)std::max(1LL, ((long long)n + per - 1) / per);
 }
 struct nest_solver1d {
     int n = 0, bc = 0, lim = 0, mode = 0;
@@ -1396,7 +1409,7 @@ struct nest_solver1d {
     // reads were reduced (upload, caller-dt stages or steps): the next
     // advance reduces them again first
     bool speeds_stale = false;
-    int step_kernel = 0;  // fused step kernel: 0 automatic, 1 LDS tile, 2 lanes
+    int step_kernel = 0;  // fused step kernel: 0 automatic, 1 LDS tile, 2 lanes, 3 lane pairs
     hipGraph_t graph = nullptr;  // kBatch device-dt steps, replayed by run()
     hipGraphExec_t graph_exec = nullptr;
     static constexpr int kBatch = 48;  // a multiple of 2 (buffers, time slots) and 3 (slot sets)
@@ -1435,7 +1448,7 @@ struct nest_solver1d {
         a.slots = slots_d + (size_t)(k % 3) * kSet;
         a.slots_rd = slots_d + (size_t)((k + 2) % 3) * kSet;
         a.slots_clr = slots_d + (size_t)((k + 1) % 3) * kSet;
-        a.segs = step1d_segs(n, step_kernel);
+        step1d_kernel_args(n, step_kernel, a);
         L().step1d(lim, a, stream);
     }
     // the speeds of the current state (in U) into the read set of step k_next
@@ -1593,7 +1606,9 @@ int nest_hip_solver1d_run(nest_solver1d* s, double t_final, double cfl, int* ste
 
 int nest_hip_solver1d_set_step_kernel(nest_solver1d* s, int kind) {
     if (!s) return fail(NEST_ERR_INVALID, "null handle");
-    if (kind < 0 || kind > 2) return fail(NEST_ERR_INVALID, "kind: 0 automatic, 1 LDS tile, 2 lanes");
+    if (kind < 0 || kind > 3) return fail(NEST_ERR_INVALID, "kind: 0 automatic, 1 LDS tile, 2 lanes, 3 lane pairs");
+    if (kind == 3 && !step1d_pairs_fit(s->n))
+        return fail(NEST_ERR_UNSUPPORTED, "the lane-pair kernel addresses the state with 32-bit byte offsets: n < 178956966");
     s->step_kernel = kind;
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
@@ -1754,7 +1769,7 @@ int nest_hip_euler1d_step_noalloc(const double* U_in, double* U_out, int n, doub
     a.slots = slots;
     a.slots_rd = slots + 2 * kNoallocSet;
     a.slots_clr = slots + kNoallocSet;
-    a.segs = step1d_segs(n, 0);  // the solver's automatic choice
+    step1d_kernel_args(n, 0, a);  // the solver's automatic choice
     L.step1d(limiter, a, st);
     L.fill1d(U_out, n, bc, nullptr, st);  // hand out a halo-complete state
     if (dt_out)

@@ -72,8 +72,10 @@ class Solver1D:
 
     def set_step_kernel(self, kind: str = "auto") -> None:
         """Kernel of the fused device-dt steps: "auto", "tile" (LDS tile per
-        block) or "lanes" (each wave on its own 56-cell segments)."""
-        check(self.L.nest_hip_solver1d_set_step_kernel(self.h, {"auto": 0, "tile": 1, "lanes": 2}[kind]))
+        block), "lanes" (each wave on its own 56-cell segments, one cell per
+        lane) or "pairs" (120-cell segments, two cells per lane)."""
+        kinds = {"auto": 0, "tile": 1, "lanes": 2, "pairs": 3}
+        check(self.L.nest_hip_solver1d_set_step_kernel(self.h, kinds[kind]))
 
     def stage(self, stage: int, dt: float) -> None:
         """One SSP-RK2 stage with the given dt (1: U1 = S(U); 2: U = (U + S(U1)) / 2)."""

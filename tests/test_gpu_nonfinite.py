@@ -53,7 +53,7 @@ def test_2d_nan_stops_device_dt_run(mode, path, when):
     sim.close()
 
 
-@pytest.mark.parametrize("kernel", ["tile", "lanes"])
+@pytest.mark.parametrize("kernel", ["tile", "lanes", "pairs"])
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_1d_nan_stops_device_dt_run(mode, kernel):
     n = 500

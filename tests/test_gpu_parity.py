@@ -57,7 +57,7 @@ def test_1d_sod_lockstep(oracle, mode, bc, lim):
     assert worst <= TOL_FAST
 
 
-STEP1D = ["tile", "lanes"]
+STEP1D = ["tile", "lanes", "pairs"]
 
 
 @pytest.mark.parametrize("kernel", STEP1D)
@@ -97,12 +97,14 @@ def test_1d_run_fused_step_sizes(oracle, bc, lim, kernel):
     the oracle's run (exact numerics) for sizes of 2 and 3 cells (one block
     whose halo wraps / mirrors / clamps onto itself), 257 (a last block of one
     cell) and 1000, each at two end times whose step counts differ in parity
-    (the state ends in either buffer), for each limiter."""
+    (the state ends in either buffer), for each limiter; 57 and 121 end
+    one-cell-per-lane and two-cells-per-lane segments one cell past a whole
+    segment."""
     # minmod on the Sod jump; MC and the unlimited scheme on a weak shock
     # tube (on the strong jump at 2-3 cells the unlimited scheme drives the
     # density negative, and the run ends with a non-finite dt on either side)
     left, right = ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1)) if lim == "minmod" else ((1.0, 0.0, 1.0), (0.8, 0.0, 0.75))
-    for n, t_probe in [(2, 0.5), (3, 0.62), (57, 0.1), (257, 0.05), (1000, 0.05)]:
+    for n, t_probe in [(2, 0.5), (3, 0.62), (57, 0.1), (121, 0.05), (257, 0.05), (1000, 0.05)]:
         U = P.sod1d(n, left=left, right=right)
         oracle.fill_1d(U, n, bc=bc)
         # end times half-way into steps j and j + 1 of the oracle's dt
@@ -811,11 +813,13 @@ def test_1d_start_advance_and_stages(oracle, bc, kernel):
 
 @pytest.mark.parametrize("bc", ["periodic", "outflow", "reflecting"])
 def test_1d_large_lane_kernel_segments(oracle, bc):
-    """The lane kernel with several segments per wave (n = 56 * 16384 + 77:
-    2 segments per wave, a ragged last wave and block), fast numerics: 4
-    device-dt steps equal the tile kernel's within 1e-12 and the oracle's
-    (exact numerics, bitwise)."""
-    n = 56 * 16384 + 77
+    """The lane kernels with several segments per wave (n = 120 * 16384 + 78:
+    3 segments of 56 cells or 2 of 120 cells per wave, a ragged last wave and
+    block; n even, so the two-cells-per-lane kernel takes its 16-byte
+    accesses, which the odd sizes of test_1d_run_fused_step_sizes do not):
+    4 device-dt steps equal the oracle's bitwise (exact numerics) and within
+    4e-12 (fast numerics)."""
+    n = 120 * 16384 + 78
     print_case("1d lanes segments", bc)
     U = P.sod1d(n)
     oracle.fill_1d(U, n, bc=bc)
@@ -827,7 +831,7 @@ def test_1d_large_lane_kernel_segments(oracle, bc):
     oracle.fill_1d(ref, n, bc=bc)
     outs = {}
     for mode in ("exact", "fast"):
-        for kernel in ("lanes", "tile"):
+        for kernel in ("lanes", "pairs", "tile"):
             s = Solver1D(n, dx, bc=bc, mode=mode)
             s.set_step_kernel(kernel)
             s.upload(U)
@@ -835,7 +839,8 @@ def test_1d_large_lane_kernel_segments(oracle, bc):
             s.advance(4)
             outs[(mode, kernel)] = s.download()
             s.close()
-    assert np.array_equal(outs[("exact", "lanes")], ref)
-    assert np.array_equal(outs[("exact", "tile")], ref)
-    got, want = outs[("fast", "lanes")].reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2]
-    assert rel_linf_vars(got, want) <= 4e-12
+    for kernel in ("lanes", "pairs", "tile"):
+        assert np.array_equal(outs[("exact", kernel)], ref), kernel
+    for kernel in ("lanes", "pairs"):
+        got, want = outs[("fast", kernel)].reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2]
+        assert rel_linf_vars(got, want) <= 4e-12, kernel
