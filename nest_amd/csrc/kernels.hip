@@ -25,6 +25,8 @@
 // (rows loaded into registers, with an LDS ring for the centre rows; 64-bit
 // addressing, used above that size). Nothing but the state itself touches
 // HBM.
+#include <atomic>
+
 #include "device_numerics.hpp"
 #include "kernels.hpp"
 
@@ -1599,9 +1601,11 @@ __device__ __forceinline__ void step1d_fold_speeds(const stage1d_args& a, unsign
 // 2..61, stage 2 on lanes 4..59, so a segment produces 56 cells. Past an
 // outflow / reflecting wall, U1 comes from the lane of the mapped cell
 // (nest_step_lane_source), momentum negated at a reflecting wall; past a
-// periodic end stage 1 sees the stencil of the cell it wraps to. Each wave
-// runs a.segs consecutive segments (one dt read per wave).
-// Arithmetic and order per cell are those of step1d_kernel.
+// periodic end stage 1 sees the stencil of the cell it wraps to. The launch
+// holds no more waves than the GPU keeps resident; wave w takes segments w,
+// w + W, w + 2W, ... (W = a.waves), so every wave runs to the end of the
+// launch with the same number of segments, give or take one, and reads dt
+// once. Arithmetic and order per cell are those of step1d_kernel.
 template <int LIM>
 __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
     __shared__ unsigned long long wmax[4];
@@ -1613,11 +1617,11 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
     double* out = a.U1;
     // U^n of segment sg's lanes (cell g0 - 4 + lane through the boundary map;
     // momentum not yet negated past a reflecting wall)
+    const int nseg = (n + 55) / 56, W = a.waves;
     double p0 = 0.0, pm = 0.0, p2 = 0.0;
     auto fetch = [&](int sg) {
-        const int g0 = (wid * a.segs + sg) * 56;
-        if (sg < a.segs && g0 < n) {
-            const int src = nest_map_index(g0 - 4 + lane, n, a.bc) + 2;
+        if (sg < nseg) {
+            const int src = nest_map_index(sg * 56 - 4 + lane, n, a.bc) + 2;
             p0 = in[src];
             pm = in[ne + src];
             p2 = in[2 * ne + src];
@@ -1625,19 +1629,18 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
     };
     // the first segment's loads go out before the dt arithmetic, which they
     // do not depend on
-    fetch(0);
+    fetch(wid);
     double dt;
     const bool go = step1d_schedule(a, wid, lane, dt) == 0;
     const double dtdx = dt / a.ps->dx;
     unsigned long long sm = 0;
-    for (int sg = 0; sg < a.segs; ++sg) {
-        const int g0 = (wid * a.segs + sg) * 56;
-        if (g0 >= n) break;
+    for (int sg = wid; sg < nseg; sg += W) {
+        const int g0 = sg * 56;
         const int g = g0 - 4 + lane;
         const bool store = lane >= 4 && lane < 60 && g < n;
         const double m = pm;
         double c0 = p0, c2 = p2;
-        fetch(sg + 1);  // the next segment's loads, in flight during this one
+        fetch(sg + W);  // the next segment's loads, in flight during this one
         if (!go) {
             // a step past the end passes the state through (stored lanes
             // hold interior cells: their loads are the cells themselves)
@@ -1701,8 +1704,8 @@ __global__ __launch_bounds__(256) void step1d_lanes_kernel(stage1d_args a) {
 // is valid on lanes 1..62, stage 2 on lanes 2..61: a segment of 128 cells
 // produces 120. Past an outflow / reflecting wall, U1 comes from the lane
 // and half of the mapped cell, momentum negated at a reflecting wall; past
-// a periodic end stage 1 sees the stencil of the cell it wraps to. Each
-// wave runs a.segs consecutive segments. WIDE (state buffers and the
+// a periodic end stage 1 sees the stencil of the cell it wraps to. Waves
+// take segments as in step1d_lanes_kernel. WIDE (state buffers and the
 // variable stride 16-byte aligned: n even): a lane moves its two cells with
 // one 16-byte access per variable (a wave's access spans whole cache lines,
 // where two 8-byte accesses each touch every other 8 bytes of twice as
@@ -1726,10 +1729,11 @@ __global__ __launch_bounds__(256) void step1d_pairs_kernel(stage1d_args a) {
     auto st = [&](unsigned o, double x) { *reinterpret_cast<double*>(out8 + o) = x; };
     // U^n of segment sg's cells A and B (through the boundary map; momentum
     // not yet negated past a reflecting wall)
+    const int nseg = (n + 119) / 120, W = a.waves;
     double pA[3] = {0.0, 0.0, 0.0}, pB[3] = {0.0, 0.0, 0.0};
     auto fetch = [&](int sg) {
-        const int g0 = (wid * a.segs + sg) * 120;
-        if (sg < a.segs && g0 < n) {
+        if (sg < nseg) {
+            const int g0 = sg * 120;
             const int gA = g0 - 4 + 2 * lane;
             if (WIDE && g0 >= 4 && g0 + 124 <= n) {
                 const unsigned oA = (unsigned)(gA + 2) * 8u;
@@ -1750,14 +1754,13 @@ __global__ __launch_bounds__(256) void step1d_pairs_kernel(stage1d_args a) {
             }
         }
     };
-    fetch(0);
+    fetch(wid);
     double dt;
     const bool go = step1d_schedule(a, wid, lane, dt) == 0;
     const double dtdx = dt / a.ps->dx;
     unsigned long long sm = 0;
-    for (int sg = 0; sg < a.segs; ++sg) {
-        const int g0 = (wid * a.segs + sg) * 120;
-        if (g0 >= n) break;
+    for (int sg = wid; sg < nseg; sg += W) {
+        const int g0 = sg * 120;
         const int gA = g0 - 4 + 2 * lane, gB = gA + 1;
         const bool mid = lane >= 2 && lane < 62;
         const bool storeA = mid && gA < n, storeB = mid && gB < n;
@@ -1767,7 +1770,7 @@ __global__ __launch_bounds__(256) void step1d_pairs_kernel(stage1d_args a) {
             A[v] = pA[v];
             B[v] = pB[v];
         }
-        fetch(sg + 1);  // the next segment's loads, in flight during this one
+        fetch(sg + W);  // the next segment's loads, in flight during this one
         const unsigned eA = (unsigned)(gA + 2) * 8u;  // cell A's byte offset (stored lanes)
         // both cells of a lane (n even: storeA == storeB)
         auto st2 = [&](unsigned o, double x, double y) {
@@ -2006,31 +2009,78 @@ void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s) {
     else stage1d_dispatch<2>(lim, a, grid, s);
 }
 
-template <bool WIDE>
-static void pairs_dispatch(int lim, const stage1d_args& a, int grid, hipStream_t s) {
+// Resident blocks (256 threads) of kernel K on the current device: its
+// occupancy per compute unit times the compute units, queried once per
+// device (0 if a query fails).
+template <void (*K)(stage1d_args)>
+static int resident_blocks_1d() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int v = cached[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, K, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    v = per_cu * cus;
+    cached[dev].store(v, std::memory_order_relaxed);
+    return v;
+}
+
+// one lane kernel launch over nseg segments: at most the resident blocks of
+// 4 waves, every wave looping over its share of the segments
+template <void (*K)(stage1d_args)>
+static void launch_lanes(stage1d_args a, int nseg, hipStream_t s) {
+    const int need = (nseg + 3) / 4;
+    const int res = resident_blocks_1d<K>();
+    const int grid = std::max(1, res > 0 ? std::min(need, res) : need);
+    a.waves = 4 * grid;
+    hipLaunchKernelGGL(K, grid, 256, 0, s, a);
+}
+
+template <int LIM>
+static void lanes_dispatch(const stage1d_args& a, hipStream_t s) {
+    if (!a.pairs) {
+        launch_lanes<step1d_lanes_kernel<LIM>>(a, (a.n + 55) / 56, s);
+        return;
+    }
+    const int nseg = (a.n + 119) / 120;
+    if ((((uintptr_t)a.U | (uintptr_t)a.U1) & 15) == 0 && (a.n & 1) == 0)
+        launch_lanes<step1d_pairs_kernel<LIM, true>>(a, nseg, s);
+    else
+        launch_lanes<step1d_pairs_kernel<LIM, false>>(a, nseg, s);
+}
+
+// The lane kernels' resident counts of limiter lim on the current device,
+// queried ahead (host queries only, no launch), so that launches inside a
+// later graph capture make none.
+void prepare_step1d(int lim) {
     switch (lim) {
-        case LIM_MC: hipLaunchKernelGGL((step1d_pairs_kernel<LIM_MC, WIDE>), grid, 256, 0, s, a); break;
-        case LIM_NONE: hipLaunchKernelGGL((step1d_pairs_kernel<LIM_NONE, WIDE>), grid, 256, 0, s, a); break;
-        default: hipLaunchKernelGGL((step1d_pairs_kernel<LIM_MINMOD, WIDE>), grid, 256, 0, s, a); break;
+        case LIM_MC:
+            (void)resident_blocks_1d<step1d_lanes_kernel<LIM_MC>>();
+            (void)resident_blocks_1d<step1d_pairs_kernel<LIM_MC, true>>();
+            (void)resident_blocks_1d<step1d_pairs_kernel<LIM_MC, false>>();
+            break;
+        case LIM_NONE:
+            (void)resident_blocks_1d<step1d_lanes_kernel<LIM_NONE>>();
+            (void)resident_blocks_1d<step1d_pairs_kernel<LIM_NONE, true>>();
+            (void)resident_blocks_1d<step1d_pairs_kernel<LIM_NONE, false>>();
+            break;
+        default:
+            (void)resident_blocks_1d<step1d_lanes_kernel<LIM_MINMOD>>();
+            (void)resident_blocks_1d<step1d_pairs_kernel<LIM_MINMOD, true>>();
+            (void)resident_blocks_1d<step1d_pairs_kernel<LIM_MINMOD, false>>();
+            break;
     }
 }
 
 void launch_step1d(int lim, const stage1d_args& a, hipStream_t s) {
-    if (a.segs > 0 && a.pairs) {  // two cells per lane: a.segs segments of 120 cells per wave
-        const int waves = (a.n + 120 * a.segs - 1) / (120 * a.segs);
-        const int grid = (waves + 3) / 4;
-        const bool wide = (((uintptr_t)a.U | (uintptr_t)a.U1) & 15) == 0 && (a.n & 1) == 0;
-        if (wide) pairs_dispatch<true>(lim, a, grid, s);
-        else pairs_dispatch<false>(lim, a, grid, s);
-        return;
-    }
-    if (a.segs > 0) {  // lane kernel: a.segs segments of 56 cells per wave, 4 waves per block
-        const int waves = (a.n + 56 * a.segs - 1) / (56 * a.segs);
-        const int grid = (waves + 3) / 4;
+    if (a.lanes) {
         switch (lim) {
-            case LIM_MC: hipLaunchKernelGGL((step1d_lanes_kernel<LIM_MC>), grid, 256, 0, s, a); break;
-            case LIM_NONE: hipLaunchKernelGGL((step1d_lanes_kernel<LIM_NONE>), grid, 256, 0, s, a); break;
-            default: hipLaunchKernelGGL((step1d_lanes_kernel<LIM_MINMOD>), grid, 256, 0, s, a); break;
+            case LIM_MC: lanes_dispatch<LIM_MC>(a, s); break;
+            case LIM_NONE: lanes_dispatch<LIM_NONE>(a, s); break;
+            default: lanes_dispatch<LIM_MINMOD>(a, s); break;
         }
         return;
     }

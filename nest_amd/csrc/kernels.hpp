@@ -19,6 +19,7 @@
     void launch_reset_speeds(nest_prob_state* P, unsigned long long* slots, int nprob, hipStream_t s); \
     void launch_stage1d(int stage, int lim, const stage1d_args& a, hipStream_t s);                  \
     void launch_step1d(int lim, const stage1d_args& a, hipStream_t s);                              \
+    void prepare_step1d(int lim);                                                                   \
     void launch_speeds1d(const stage1d_args& a, hipStream_t s);                                     \
     void launch_fill1d(double* U, int n, int bc, const nest_prob_state* ps, hipStream_t s);         \
     void launch_adj_y(int lim, const adj_args& a, int grid, int block, hipStream_t s);              \
@@ -100,11 +101,14 @@ struct stage1d_args {
     const unsigned long long* slots_rd;
     unsigned long long* slots_clr;
     int par;
-    // fused step kernel: 0 the LDS-tile kernel (256 cells per block); > 0
-    // a lane kernel with this many segments per wave: of 56 cells (one cell
-    // per lane), or of 120 cells when pairs is set (two cells per lane)
-    int segs;
+    // fused step kernel: lanes 0 the LDS-tile kernel (256 cells per block),
+    // 1 a lane kernel: one cell per lane in 56-cell segments, or (pairs) two
+    // cells per lane in 120-cell segments. waves: the lane kernel launch's
+    // wave count (set by launch_step1d); wave w takes segments w, w + waves,
+    // w + 2 waves, ...
+    int lanes;
     int pairs;
+    int waves;
 };
 
 NEST_DECLARE_LAUNCHERS(nest_exact)

@@ -56,6 +56,7 @@ struct launch_table {
     void (*reset_speeds)(nest_prob_state*, unsigned long long*, int, hipStream_t);
     void (*stage1d)(int, int, const stage1d_args&, hipStream_t);
     void (*step1d)(int, const stage1d_args&, hipStream_t);
+    void (*prepare_step1d)(int);
     void (*speeds1d)(const stage1d_args&, hipStream_t);
     void (*fill1d)(double*, int, int, const nest_prob_state*, hipStream_t);
     void (*adj_y)(int, const adj_args&, int, int, hipStream_t);
@@ -68,7 +69,7 @@ const launch_table kExact = {nest_exact::launch_stage2d,     nest_exact::launch_
                              nest_exact::step2d_resident_blocks, nest_exact::launch_speeds2d,
                              nest_exact::launch_fill_regions, nest_exact::launch_finalize,
                              nest_exact::launch_reset_speeds, nest_exact::launch_stage1d,
-                             nest_exact::launch_step1d,       nest_exact::launch_speeds1d,
+                             nest_exact::launch_step1d, nest_exact::prepare_step1d,       nest_exact::launch_speeds1d,
                              nest_exact::launch_fill1d,
                              nest_exact::launch_adj_y,        nest_exact::launch_adj_x,
                              nest_exact::launch_pack};
@@ -76,7 +77,7 @@ const launch_table kFast = {nest_fast::launch_stage2d,     nest_fast::launch_ste
                             nest_fast::step2d_resident_blocks, nest_fast::launch_speeds2d,
                             nest_fast::launch_fill_regions, nest_fast::launch_finalize,
                             nest_fast::launch_reset_speeds, nest_fast::launch_stage1d,
-                            nest_fast::launch_step1d,       nest_fast::launch_speeds1d,
+                            nest_fast::launch_step1d, nest_fast::prepare_step1d,       nest_fast::launch_speeds1d,
                             nest_fast::launch_fill1d,
                             nest_fast::launch_adj_y,        nest_fast::launch_adj_x,
                             nest_fast::launch_pack};
@@ -1371,26 +1372,21 @@ int nest_hip_sim2d_rollout_info(const nest_sim2d* s, int* nsteps, int* interval,
 // The pair kernel addresses the state (3 (n + 4) doubles) with 32-bit byte
 // offsets.
 static bool step1d_pairs_fit(int n) { return 24ULL * ((unsigned long long)n + 4) < (1ULL << 32); }
-// The fused 1D step kernel for n cells: kind 1 the LDS-tile kernel (segs =
-// 0); kind 2 the lane kernel (one cell per lane, 56-cell segments), kind 3
-// the pair kernel (two cells per lane, 120-cell segments), each wave running
-// segs segments (about 16k waves per launch, so each wave's dt read and the
-// block atomics are amortised). Kind 0 automatic: the pair kernel from 2^19
-// cells (where it fits), the lane kernel below. Measured per step (fast /
-// exact numerics): 2^24 cells 186.9 / 322.4 us against 204.6 / 350.8 us for
-// the lane kernel, 2^19 12.0 / 17.1 against 13.1 / 18.4, 2^18 8.5 / 11.7
-// against 8.6 / 11.3, 1e5 6.2 / 6.9 against 6.3 / 6.8, 1000 5.8 / 6.7
-// against 5.3 / 5.8 (the lane kernel: 7.1 against 10.3 us at 1e5 and 262
-// against 406 us at 2^24 for the tile kernel).
+// The fused 1D step kernel for n cells: kind 1 the LDS-tile kernel; kind 2
+// the lane kernel (one cell per lane, 56-cell segments), kind 3 the pair
+// kernel (two cells per lane, 120-cell segments), each with no more waves
+// than stay resident, every wave looping over its share of the segments.
+// Kind 0 automatic: the pair kernel from 2^19 cells (where it fits), the
+// lane kernel below. Measured per step (fast / exact numerics): 2^24 cells
+// 186.9 / 322.4 us against 204.6 / 350.8 us for the lane kernel, 2^19 12.0 /
+// 17.1 against 13.1 / 18.4, 2^18 8.5 / 11.7 against 8.6 / 11.3, 1e5 6.2 /
+// 6.9 against 6.3 / 6.8, 1000 5.8 / 6.7 against 5.3 / 5.8 (the lane kernel:
+// 7.1 against 10.3 us at 1e5 and 262 against 406 us at 2^24 for the tile
+// kernel).
 static void step1d_kernel_args(int n, int kind, stage1d_args& a) {
     if (kind == 0) kind = n >= (1 << 19) && step1d_pairs_fit(n) ? 3 : 2;
+    a.lanes = kind != 1;
     a.pairs = kind == 3;
-    if (kind == 1) {
-        a.segs = 0;
-        return;
-    }
-    const long long per = (a.pairs ? 120LL : 56LL) * 16384;
-    a.segs = (int)std::max(1LL, ((long long)n + per - 1) / per);
 }
 struct nest_solver1d {
     int n = 0, bc = 0, lim = 0, mode = 0;
@@ -1505,6 +1501,7 @@ int nest_hip_solver1d_create(int n, double dx, double gamma, int bc, int limiter
         return fail(NEST_ERR_HIP, "stream/memory setup failed");
     }
     s->stream = s->own_stream;
+    s->L().prepare_step1d(limiter);  // before any graph capture of the step
     *out = s;
     return NEST_OK;
 }
