@@ -98,8 +98,8 @@ int nest_hip_solver1d_set_stream(nest_solver1d* s, void* stream);
    automatic (kind 3 from 2^19 cells, kind 2 below), 1
    the LDS-tile kernel (256 cells per block, U1 in LDS), 2 the lane kernel
    (each wave on its own segments of 56 cells, U1 in registers, no barrier),
-   3 the lane kernel with two cells per lane (segments of 120 cells; n below
-   178956966, else NEST_ERR_UNSUPPORTED). All compute the same arithmetic.
+   3 the lane kernel with two cells per lane (segments of 120 cells; n at
+   most 178956966, else NEST_ERR_UNSUPPORTED). All compute the same arithmetic.
    No counterpart in the reference. */
 int nest_hip_solver1d_set_step_kernel(nest_solver1d* s, int kind);
 /* One SSP-RK2 stage with the given dt, the per-stage granularity of a

@@ -1605,7 +1605,7 @@ int nest_hip_solver1d_set_step_kernel(nest_solver1d* s, int kind) {
     if (!s) return fail(NEST_ERR_INVALID, "null handle");
     if (kind < 0 || kind > 3) return fail(NEST_ERR_INVALID, "kind: 0 automatic, 1 LDS tile, 2 lanes, 3 lane pairs");
     if (kind == 3 && !step1d_pairs_fit(s->n))
-        return fail(NEST_ERR_UNSUPPORTED, "the lane-pair kernel addresses the state with 32-bit byte offsets: n < 178956966");
+        return fail(NEST_ERR_UNSUPPORTED, "the lane-pair kernel addresses the state with 32-bit byte offsets: n <= 178956966");
     s->step_kernel = kind;
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);

@@ -844,3 +844,29 @@ def test_1d_large_lane_kernel_segments(oracle, bc):
     for kernel in ("lanes", "pairs"):
         got, want = outs[("fast", kernel)].reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2]
         assert rel_linf_vars(got, want) <= 4e-12, kernel
+
+
+def test_1d_pair_kernel_size_limit():
+    """The two-cells-per-lane kernel addresses the state with 32-bit byte
+    offsets (3 (n + 4) doubles below 4 GiB): past that size an explicit
+    request is refused (NEST_ERR_UNSUPPORTED) and the automatic choice takes
+    the one-cell lane kernel, which one 2-step advance runs (no NaN, time
+    advanced). Just below the limit the request is accepted."""
+    from nest_amd import _lib
+    n_max = (1 << 32) // 24 - 4  # largest n with 24 (n + 4) < 2^32
+    print_case("1d pair kernel size limit", n_max)
+    s = Solver1D(n_max, 1.0 / n_max, bc="outflow", mode="fast")
+    s.set_step_kernel("pairs")
+    s.close()
+    n = n_max + 1
+    s = Solver1D(n, 1.0 / n, bc="outflow", mode="fast")
+    with pytest.raises(_lib.NestError) as e:
+        s.set_step_kernel("pairs")
+    assert e.value.code == 5
+    s.set_step_kernel("auto")
+    s.upload(P.sod1d(n))
+    s.start(0.4)
+    s.advance(2)
+    st = s.status()
+    assert st["steps"] == 2 and st["done"] == 0 and st["t"] > 0.0, st
+    s.close()
