@@ -10,7 +10,10 @@ stepped as examples/sod2d/sod2d.cpp:318-356 with the build-defined RK2):
     asserted equal to the oracle's), then device-dt steps by step count;
   * 2D batches of independent problems of uneven sizes under a drawn tiling:
     device-dt steps;
-  * 1D: device-dt steps of a drawn step kernel.
+  * 1D: device-dt steps of a drawn step kernel;
+  * the discrete adjoint of a K-step rollout (checkpoint interval and
+    adjoint kernel tiling drawn): its forward against the oracle, its
+    reverse sweep by the dot-product identity against dual numbers.
 Exact numerics must be bitwise equal, fast numerics within 1e-12 relative
 L-infinity per variable (dt within 1e-13 relative).
 The runs are derandomized: every run draws the same examples, so a failure
@@ -183,3 +186,55 @@ def test_fuzz_1d(oracle, n, bc, lim, kernel, nsteps, mode, seed):
             _same(got.reshape(3, -1)[:, 2:-2], ref.reshape(3, -1)[:, 2:-2], mode, "1d")
     finally:
         s.close()
+
+
+@SETTINGS
+@given(nx=st.integers(2, 80), ny=st.integers(2, 80), px=st.integers(1, 3), py=st.integers(1, 3),
+       bcx=st.sampled_from(["periodic", "outflow"]), bcy=st.sampled_from(["periodic", "outflow"]),
+       lim=st.sampled_from(LIMS), mode=st.sampled_from(MODES), K=st.integers(1, 8),
+       interval=st.sampled_from([0, 1, 2, 3, -1, 100]),
+       rows_y=st.sampled_from(autotune.ADJ_ROWS), waves_y=st.sampled_from(autotune.WAVES),
+       rows_x=st.sampled_from(autotune.ADJ_ROWS), waves_x=st.sampled_from(autotune.WAVES),
+       seed=st.integers(0, 2**31 - 1))
+def test_fuzz_adjoint_rollout(oracle, nx, ny, px, py, bcx, bcy, lim, mode, K, interval, rows_y, waves_y, rows_x,
+                              waves_x, seed):
+    """The discrete adjoint of a K-step fixed-dt rollout (checkpoint interval,
+    adjoint kernel tiling and the rest drawn): its final state equals the
+    oracle's K steps (exact bitwise, fast within 1e-11), and its reverse
+    sweep satisfies the dot-product identity <J^T w, v> = <w, J v> within
+    1e-11 relative, J v from the oracle's dual-number JVP of the K-step map
+    (the adjoint's own tests pin it entry by entry on fixed cases)."""
+    torch = pytest.importorskip("torch")
+    npx, npy = max(1, min(px, nx // 2)), max(1, min(py, ny // 2))
+    print_case("fuzz adjoint", (nx, ny, npx, npy), bcx, bcy, lim, mode, K, interval,
+               (rows_y, waves_y, rows_x, waves_x), seed)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    m = MeshSpec(nx, ny, npx, npy, bcx=bcx, bcy=bcy)
+    sim = Sim2D.mesh(nx, ny, npx, npy, dx, dy, bcx=bcx, bcy=bcy, limiter=lim, mode=mode)
+    try:
+        sim.set_adjoint_tiling(rows_y, waves_y, rows_x, waves_x)
+        U = sim.scatter(random_state_2d(seed, nx, ny, mild=lim == "none"))
+        oracle.mesh_fill(U, m)
+        dt = 0.5 * oracle.mesh_dt(U, m, dx, dy, 0.4)
+        sim.upload(U)
+        sim.rollout(K, dt, interval)
+        final = sim.gather(sim.download())
+        r = np.random.default_rng(seed ^ 0x5EED)
+        v = r.standard_normal((4, nx, ny))
+        w = r.standard_normal((4, nx, ny))
+        Uc, dU = U.copy(), sim.scatter(v)
+        oracle.mesh_rk2_jvp(Uc, dU, m, K, dt, dx, dy, lim=lim)
+        if mode == "exact":
+            assert np.array_equal(final, sim.gather(Uc)), "rollout state differs from the oracle"
+        else:
+            assert rel_linf_vars(final, sim.gather(Uc)) <= 1e-11
+        Wd = torch.from_numpy(sim.scatter(w)).to("cuda")
+        torch.cuda.synchronize()
+        sim.rollout_adjoint(Wd.data_ptr())
+        sim.synchronize()
+        a = sim.gather(Wd.cpu().numpy())
+        lhs, rhs = float(np.sum(a * v)), float(np.sum(w * sim.gather(dU)))
+        err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+        assert err <= 1e-11, (lhs, rhs, err)
+    finally:
+        sim.close()
