@@ -99,19 +99,21 @@ def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 5, cac
                  keep_all: bool = False):
     """Tune the adjoint kernels' tiling of `sim` in place: coordinate descent
     (two passes) over rows and waves per block of the Y and X kernels. One
-    16-step rollout from the handle's own state is recorded first, stored the
-    way the caller's rollouts will be: every state kept (`keep_all`; the
-    reverse runs alone), or checkpoints every 4 steps (recomputes share the
-    device with the reverse). Each candidate is then timed as the fastest of
-    `reps` reverse sweeps of it, with the handle's stage scratch as the
-    cotangent: the tiling changes the reverse sweep only, and timing the
-    forward with it hid differences of a few percent (an 8-step forward +
-    reverse per trial picked 20 x 20 rows at 1024^2, 8% slower per gradient
-    than 10 x 16). A candidate replaces the current choice only when it is
-    1% faster (differences below that are timing noise, and a noise-driven
-    switch of the X kernel to one wave per block cost the checkpointed
-    gradient 1%). Returns {"rows_y", "waves_y", "rows_x", "waves_x",
-    "ms_per_step"} (ms per reversed step)."""
+    16-step rollout from the handle's own state is recorded first, with
+    checkpoints every 2 steps: its reverse sweeps recompute intervals on the
+    auxiliary stream while they run, as checkpointed gradients do. Each
+    candidate is then timed as the fastest of `reps` reverse sweeps, with the
+    handle's stage scratch as the cotangent: the tiling changes the reverse
+    sweep only, and timing the forward with it hid differences of a few
+    percent (an 8-step forward + reverse per trial picked 20 x 20 rows at
+    1024^2, 8% slower per gradient than 10 x 16). A tiling that is fast
+    alone can lose beside the recompute: one wave per block for the X kernel
+    timed best on reverses with nothing to recompute, and the X kernel then
+    took 36.7 against 33.2 us per launch in a checkpointed gradient (every
+    state kept gained nothing from it). A candidate replaces the current
+    choice only when it is 1% faster (timing noise). `keep_all` selects the
+    cache entry of the caller's storage. Returns {"rows_y", "waves_y",
+    "rows_x", "waves_x", "ms_per_step"} (ms per reversed step)."""
     classes = size_classes(sizes)
     key = ("adjoint-rollout-kept|" if keep_all else "adjoint-rollout|") + signature(sizes, classes, arch)
     path = _cache_path(cache)
@@ -122,7 +124,7 @@ def tune_adjoint(sim, sizes, dt: float, arch: str = "gfx950", reps: int = 5, cac
         return c
     _, U1 = sim.device_ptrs()
     nsteps = 16
-    sim.rollout(nsteps, dt, -1 if keep_all else 4)
+    sim.rollout(nsteps, dt, 2)
 
     def run(c):
         sim.set_adjoint_tiling(c["rows_y"], c["waves_y"], c["rows_x"], c["waves_x"])
