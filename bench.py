@@ -528,13 +528,16 @@ def kh_leg(mode: str, steps: int = 100, nadj: int = 64, adj_steps: int = 200):
             density_misfit(rb(U0), tgt, area)
         rb.interval = 0
 
-    def timed(fn):
+    def timed(fn, reps=3):  # median of `reps` runs after one untimed run
         fn()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t
+        out = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            out.append(time.perf_counter() - t)
+        return sorted(out)[reps // 2]
 
     t_b = timed(batched)
     t_f = timed(forward)
