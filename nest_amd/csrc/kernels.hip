@@ -784,7 +784,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 4))) voi
         // or done flag) and clears the slot set the next step writes. No
         // wave reads what another wave of the launch writes: the time comes
         // from tn[par], the record goes to tn[par ^ 1], and a done flag set
-        // here gives every item the same verdict (pass the state through).
+        // here gives every item the same outcome (pass the state through).
         double dt = a.host_dt;
         bool pass = false;
         if (!(a.host_dt > 0.0)) {
