@@ -45,6 +45,38 @@ def test_error_reporting_without_device():
     assert rc == 1 and "2 (reflecting)" in _lib.last_error()
 
 
+def test_invalid_arguments_rejected_before_device_work():
+    """Every entry point below rejects its bad argument with NEST_ERR_INVALID
+    and a message, before any device call (so the checks run on a host with
+    no GPU): no exception and no crash crosses the C ABI."""
+    L = _lib.load()
+    h = C.c_void_p()
+    buf = (C.c_double * 8)()
+    other = (C.c_double * 8)()
+    cases = [
+        ("1d gamma", lambda: L.nest_hip_solver1d_create(16, 0.1, 1.0, 0, 0, 0, C.byref(h)), "gamma > 1"),
+        ("1d bc", lambda: L.nest_hip_solver1d_create(16, 0.1, 1.4, 3, 0, 0, C.byref(h)), "bc in {0,1,2}"),
+        ("1d limiter", lambda: L.nest_hip_solver1d_create(16, 0.1, 1.4, 0, 7, 0, C.byref(h)), "limiter"),
+        ("1d out", lambda: L.nest_hip_solver1d_create(16, 0.1, 1.4, 0, 0, 0, None), "out is null"),
+        ("1d kernel on null handle", lambda: L.nest_hip_solver1d_set_step_kernel(None, 3), "null handle"),
+        ("2d dx", lambda: L.nest_hip_mesh2d_create(8, 8, 1, 1, 0.0, 0.1, 1.4, 0, 0, 0, 0, C.byref(h)), ""),
+        ("batch empty", lambda: L.nest_hip_batch2d_create(0, None, None, None, None, 1.4, 0, 0, 0, C.byref(h)), ""),
+        ("noalloc aliasing", lambda: L.nest_hip_euler1d_step_noalloc(buf, buf, 4, 0.25, 1.4, 0, 0, 0, 0.4, 0.0, 1.0,
+                                                                     None, None), "distinct"),
+        ("noalloc n", lambda: L.nest_hip_euler1d_step_noalloc(buf, other, 1, 0.25, 1.4, 0, 0, 0, 0.4, 0.0, 1.0,
+                                                              None, None), "n must be in"),
+        ("noalloc cfl", lambda: L.nest_hip_euler1d_step_noalloc(buf, other, 4, 0.25, 1.4, 0, 0, 0, 0.0, 0.0, 1.0,
+                                                                None, None), "cfl > 0"),
+    ]
+    for name, call, msg in cases:
+        h.value = None
+        rc = call()
+        err = _lib.last_error()
+        assert rc == 1, (name, rc, err)
+        assert err and msg in err, (name, err)
+        assert not h.value, name  # no handle handed out
+
+
 def test_adjoint_rollout_rejects_reflecting_before_device_work():
     from nest_amd.adjoint import Rollout
     for make in (lambda: Rollout.mesh(16, 16, bc="reflecting"), lambda: Rollout.batch(2, 16, 16, bc="reflecting")):
